@@ -1,0 +1,428 @@
+// egs_api.hip -- C-ABI of libegs.so: device queries and the one-game solver object.
+//
+// The solver object owns one device workspace.  The round loop keeps the frontier size
+// on the device: rounds are enqueued in batches and the host reads one counter per
+// batch through pinned memory, one batch behind, so the GPU never waits for the host
+// between rounds (the reference blocks on thrust::remove's count twice per round,
+// src/dev_EG_alg.cu:346-373).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "../../include/egs.h"
+#include "egs_device.h"
+#include "egs_internal.h"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(EGS_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),   \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+int check_gfx950(int device) {
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, device));
+    if (strncmp(p.gcnArchName, "gfx950", 6) != 0)
+        return fail(EGS_E_HIP, "device %d is %s; libegs.so is built for gfx950 only", device,
+                    p.gcnArchName);
+    return EGS_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Loop bound of src/egsolver.c:64-90: a negative request means n*m*MG+1, saturated to
+// LONG_MAX-1 on overflow.
+int64_t egs_max_loop_c(int64_t m, int64_t n, int64_t mg, int64_t maxloop) {
+    if (maxloop >= 0) return maxloop;
+    if (mg > (INT64_MAX - 1) / (n * m)) return INT64_MAX - 1;
+    return n * m * mg + 1;
+}
+
+int pow2ceil(int x) {
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+}  // namespace
+
+struct egs_solver {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int32_t n_cap = 0;
+    int64_t m_cap = 0;
+    void *block = nullptr;
+    size_t block_bytes = 0;
+    int *d_ptr = nullptr, *d_cptr = nullptr, *d_cpred = nullptr, *d_val = nullptr;
+    int *d_stamp = nullptr, *d_q[2] = {nullptr, nullptr}, *d_big = nullptr, *d_huge = nullptr;
+    int *d_ring = nullptr, *d_rounds = nullptr, *d_misc = nullptr, *d_tile = nullptr, *d_fill = nullptr;
+    int2 *d_edges = nullptr;
+    unsigned long long *d_stats = nullptr, *d_statsum = nullptr;
+    int *h_probe = nullptr;   // pinned: per-batch counter probes + misc readbacks
+    hipEvent_t ev_probe[2] = {nullptr, nullptr};
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+
+    egs::DevGame g{};
+    egs::LaunchShape sh{};
+    bool loaded = false, prepared = false;
+    int max_outdeg = 0;
+    int64_t next_round = 1;
+    int64_t phase_cap = 0;
+    int batch_rounds = 32;
+    const volatile uint32_t *stop_flag = nullptr;
+    egs_stats st{};
+};
+
+extern "C" {
+
+const char *egs_last_error(void) { return g_err; }
+
+int egs_device_query(int device, egs_device_info *out) {
+    if (!out) return fail(EGS_E_ARG, "egs_device_query: null output");
+    memset(out, 0, sizeof(*out));
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    out->device_count = count;
+    if (device < 0 || device >= count)
+        return fail(EGS_E_ARG, "device %d not available (%d devices)", device, count);
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, device));
+    snprintf(out->name, sizeof(out->name), "%s", p.name);
+    snprintf(out->arch, sizeof(out->arch), "%s", p.gcnArchName);
+    out->warp_size = p.warpSize;
+    out->max_threads_per_block = p.maxThreadsPerBlock;
+    out->multiprocessors = p.multiProcessorCount;
+    out->total_mem = (int64_t)p.totalGlobalMem;
+    out->shared_mem_per_block = (int64_t)p.sharedMemPerBlock;
+    out->clock_khz = p.clockRate;
+    out->ecc = p.ECCEnabled;
+    int dv = 0, rv = 0;
+    HIP_TRY(hipDriverGetVersion(&dv));
+    HIP_TRY(hipRuntimeGetVersion(&rv));
+    out->driver_version = dv;
+    out->runtime_version = rv;
+    return EGS_OK;
+}
+
+int egs_device_reset(int device) {
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipDeviceReset());
+    return EGS_OK;
+}
+
+int egs_solver_destroy(egs_solver *s) {
+    if (!s) return EGS_OK;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->block) (void)hipFree(s->block);
+    if (s->h_probe) (void)hipHostFree(s->h_probe);
+    for (auto &e : s->ev_probe) if (e) (void)hipEventDestroy(e);
+    if (s->ev_a) (void)hipEventDestroy(s->ev_a);
+    if (s->ev_b) (void)hipEventDestroy(s->ev_b);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return EGS_OK;
+}
+
+int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out) {
+    if (!out) return fail(EGS_E_ARG, "egs_solver_create: null output");
+    *out = nullptr;
+    if (n_cap < 1 || m_cap < 1 || m_cap > 0x7fffffffLL)
+        return fail(EGS_E_ARG, "egs_solver_create: need 1 <= n_cap and 1 <= m_cap < 2^31 (got %d, %lld)",
+                    n_cap, (long long)m_cap);
+    int rc = check_gfx950(device);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(device));
+    egs_solver *s = new egs_solver();
+    s->device = device;
+    s->n_cap = n_cap;
+    s->m_cap = m_cap;
+    const size_t n1 = (size_t)n_cap + 1;
+    const int tiles = (n_cap + 4095) / 4096 + 1;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
+    size_t o_ptr = take(4 * n1), o_edges = take(8 * (size_t)m_cap), o_cptr = take(4 * n1),
+           o_cpred = take(4 * (size_t)m_cap), o_val = take(4 * n1), o_stamp = take(4 * n1),
+           o_q0 = take(4 * n1), o_q1 = take(4 * n1), o_big = take(4 * n1), o_huge = take(4 * n1),
+           o_ring = take(4 * 16), o_rounds = take(4 * 16), o_misc = take(8 * 16),
+           o_tile = take(4 * (size_t)tiles), o_fill = take(4 * n1),
+           o_stats = take(8 * 4 * (size_t)egs::kStatSlots), o_statsum = take(8 * 8);
+    s->block_bytes = off;
+    if (hipMalloc(&s->block, off) != hipSuccess) {
+        egs_solver_destroy(s);
+        return fail(EGS_E_NOMEM, "hipMalloc of %zu bytes failed", off);
+    }
+    char *b = (char *)s->block;
+    s->d_ptr = (int *)(b + o_ptr);
+    s->d_edges = (int2 *)(b + o_edges);
+    s->d_cptr = (int *)(b + o_cptr);
+    s->d_cpred = (int *)(b + o_cpred);
+    s->d_val = (int *)(b + o_val);
+    s->d_stamp = (int *)(b + o_stamp);
+    s->d_q[0] = (int *)(b + o_q0);
+    s->d_q[1] = (int *)(b + o_q1);
+    s->d_big = (int *)(b + o_big);
+    s->d_huge = (int *)(b + o_huge);
+    s->d_ring = (int *)(b + o_ring);
+    s->d_rounds = (int *)(b + o_rounds);
+    s->d_misc = (int *)(b + o_misc);
+    s->d_tile = (int *)(b + o_tile);
+    s->d_fill = (int *)(b + o_fill);
+    s->d_stats = (unsigned long long *)(b + o_stats);
+    s->d_statsum = (unsigned long long *)(b + o_statsum);
+    if (hipHostMalloc((void **)&s->h_probe, 256, hipHostMallocDefault) != hipSuccess) {
+        egs_solver_destroy(s);
+        return fail(EGS_E_NOMEM, "hipHostMalloc failed");
+    }
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&s->ev_probe[0]) != hipSuccess || hipEventCreate(&s->ev_probe[1]) != hipSuccess ||
+        hipEventCreate(&s->ev_a) != hipSuccess || hipEventCreate(&s->ev_b) != hipSuccess) {
+        egs_solver_destroy(s);
+        return fail(EGS_E_HIP, "stream/event creation failed");
+    }
+    *out = s;
+    return EGS_OK;
+}
+
+int egs_solver_load(egs_solver *s, int mem, int32_t n, int32_t n0, int64_t m, int32_t mg,
+                    const int32_t *ptr, const int32_t *succ, const int32_t *w) {
+    if (!s) return fail(EGS_E_ARG, "null solver");
+    if (n < 1 || n0 < 0 || n0 > n || m < 1 || mg < 0 || !ptr || !succ || !w)
+        return fail(EGS_E_ARG, "egs_solver_load: bad game (n=%d n0=%d m=%lld mg=%d)", n, n0,
+                    (long long)m, mg);
+    if (n > s->n_cap || m > s->m_cap)
+        return fail(EGS_E_CAPACITY, "game (n=%d, m=%lld) exceeds solver capacity (%d, %lld)", n,
+                    (long long)m, s->n_cap, (long long)s->m_cap);
+    HIP_TRY(hipSetDevice(s->device));
+    if (mem == EGS_MEM_HOST) {
+        // validate the CSR on the host before anything reaches the device
+        if (ptr[0] != 0 || ptr[n] != m)
+            return fail(EGS_E_ARG, "CSR offsets must run from 0 to m (got %d..%d)", ptr[0], ptr[n]);
+        int mx = 0;
+        for (int v = 0; v < n; v++) {
+            int d = ptr[v + 1] - ptr[v];
+            if (d < 1) return fail(EGS_E_ARG, "vertex %d has no out-edge", v);
+            mx = std::max(mx, d);
+        }
+        for (int64_t e = 0; e < m; e++)
+            if (succ[e] < 0 || succ[e] >= n) return fail(EGS_E_ARG, "edge %lld: successor %d out of range", (long long)e, succ[e]);
+        s->max_outdeg = mx;
+        int2 *packed = (int2 *)malloc(sizeof(int2) * (size_t)m);
+        if (!packed) return fail(EGS_E_NOMEM, "host staging allocation failed");
+        for (int64_t e = 0; e < m; e++) packed[e] = make_int2(succ[e], w[e]);
+        hipError_t e1 = hipMemcpy(s->d_ptr, ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice);
+        hipError_t e2 = hipMemcpy(s->d_edges, packed, sizeof(int2) * (size_t)m, hipMemcpyHostToDevice);
+        free(packed);
+        HIP_TRY(e1);
+        HIP_TRY(e2);
+    } else if (mem == EGS_MEM_DEVICE) {
+        // device inputs are trusted to be a valid CSR of this shape (the caller owns them)
+        HIP_TRY(hipMemcpyAsync(s->d_ptr, ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToDevice, s->stream));
+        HIP_TRY(egs::launch_pack(succ, w, s->d_edges, m, s->stream));
+        HIP_TRY(egs::launch_degree_stats(s->d_ptr, n, s->d_misc, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->h_probe + 8, s->d_misc, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        s->max_outdeg = s->h_probe[8];
+    } else {
+        return fail(EGS_E_ARG, "mem must be EGS_MEM_HOST or EGS_MEM_DEVICE");
+    }
+    egs::DevGame &g = s->g;
+    g.n = n;
+    g.n0 = n0;
+    g.mg = mg;
+    g.m = m;
+    g.ptr = s->d_ptr;
+    g.edges = s->d_edges;
+    g.cptr = s->d_cptr;
+    g.cpred = s->d_cpred;
+    g.val = s->d_val;
+    g.stamp = s->d_stamp;
+    g.queue[0] = s->d_q[0];
+    g.queue[1] = s->d_q[1];
+    g.big = s->d_big;
+    g.huge = s->d_huge;
+    g.ring = s->d_ring;
+    g.stats = s->d_stats;
+    g.rounds_done = s->d_rounds;
+    s->loaded = true;
+    s->prepared = false;
+    return EGS_OK;
+}
+
+static int pick_shape(egs_solver *s, const egs_options *o) {
+    const egs::DevGame &g = s->g;
+    egs::LaunchShape &sh = s->sh;
+    egs_options def{};
+    if (!o) o = &def;
+    const double avg = (double)g.m / (double)g.n;
+    int G = o->group;
+    if (G == 0) G = std::min(16, pow2ceil((int)(avg + 0.999)));
+    if (G < 1 || G > 64 || (G & (G - 1)))
+        return fail(EGS_E_ARG, "group must be a power of two in 1..64 (got %d)", G);
+    sh.group = G;
+    sh.big_deg = o->big_deg > 0 ? o->big_deg : std::max(64, 8 * G);
+    sh.huge_deg = o->huge_deg > 0 ? o->huge_deg : 4096;
+    if (sh.huge_deg < sh.big_deg) sh.huge_deg = sh.big_deg;
+    sh.grid = o->grid > 0 ? o->grid : 2048;
+    if (sh.grid > egs::kStatSlots) sh.grid = egs::kStatSlots;
+    sh.use_wave = s->max_outdeg > sh.big_deg;
+    sh.use_block = s->max_outdeg > sh.huge_deg;
+    sh.wave_grid = 1024;
+    sh.block_grid = 512;
+    s->batch_rounds = o->batch_rounds > 0 ? o->batch_rounds : 32;
+    s->phase_cap = egs_max_loop_c(g.m, g.n, g.mg, o->max_phases);
+    s->stop_flag = o->stop_flag;
+    return EGS_OK;
+}
+
+int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
+    if (!s || !s->loaded) return fail(EGS_E_STATE, "egs_solver_prepare: no game loaded");
+    HIP_TRY(hipSetDevice(s->device));
+    int rc = pick_shape(s, opt);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(s->ev_a, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_stats, 0, sizeof(unsigned long long) * 4 * egs::kStatSlots, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_rounds, 0, sizeof(int), s->stream));
+    HIP_TRY(egs::launch_transpose(s->g, s->d_tile, s->d_fill, s->stream));
+    HIP_TRY(egs::launch_init(s->g, s->stream));
+    HIP_TRY(hipEventRecord(s->ev_b, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_probe + 9, s->d_ring + 1, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, s->ev_a, s->ev_b));
+    memset(&s->st, 0, sizeof(s->st));
+    s->st.ms_prepare = ms;
+    s->st.initial_active = s->h_probe[9];
+    s->next_round = 1;
+    s->prepared = true;
+    return EGS_OK;
+}
+
+static int collect_stats(egs_solver *s) {
+    HIP_TRY(egs::launch_sum_stats(s->g, egs::kStatSlots, s->d_statsum, s->stream));
+    unsigned long long *h = (unsigned long long *)(s->h_probe + 16);   // 8-byte aligned
+    HIP_TRY(hipMemcpyAsync(h, s->d_statsum, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_probe + 10, s->d_rounds, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_probe + 11, s->d_ring + (s->next_round % 3), sizeof(int),
+                           hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->st.processed = (int64_t)h[0];
+    s->st.edges_relaxed = (int64_t)h[1];
+    s->st.lifted = (int64_t)h[2];
+    s->st.pred_visits = (int64_t)h[3];
+    s->st.phases = s->h_probe[10];
+    s->st.converged = s->h_probe[11] == 0;
+    return EGS_OK;
+}
+
+int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
+    if (!s || !s->prepared) return fail(EGS_E_STATE, "egs_solver_iterate: call egs_solver_prepare first");
+    HIP_TRY(hipSetDevice(s->device));
+    int64_t limit = s->phase_cap - (s->next_round - 1);
+    if (max_phases >= 0 && max_phases < limit) limit = max_phases;
+    if (s->next_round + limit > 0x7ffffff0LL) limit = 0x7ffffff0LL - s->next_round;
+    HIP_TRY(hipEventRecord(s->ev_a, s->stream));
+    int64_t launched = 0;
+    int pending = -1;     // probe slot in flight
+    bool done = false;
+    int batch = 4;        // grow towards batch_rounds so short solves do not overshoot much
+    while (!done && launched < limit) {
+        const int nb = (int)std::min<int64_t>(batch, limit - launched);
+        for (int k = 0; k < nb; k++) {
+            HIP_TRY(egs::launch_round(s->g, s->sh, (int)s->next_round, s->stream));
+            s->next_round++;
+        }
+        launched += nb;
+        const int slot = (pending + 1) & 1;
+        HIP_TRY(hipMemcpyAsync(s->h_probe + slot, s->d_ring + (s->next_round % 3), sizeof(int),
+                               hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipEventRecord(s->ev_probe[slot], s->stream));
+        if (pending >= 0) {
+            HIP_TRY(hipEventSynchronize(s->ev_probe[pending]));
+            if (s->h_probe[pending] == 0) done = true;
+        }
+        pending = slot;
+        if (s->stop_flag && *s->stop_flag) break;
+        if (batch < s->batch_rounds) batch = std::min(s->batch_rounds, batch * 2);
+    }
+    HIP_TRY(hipEventRecord(s->ev_b, s->stream));
+    int rc = collect_stats(s);
+    if (rc) return rc;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, s->ev_a, s->ev_b));
+    s->st.ms_iterate += ms;
+    if (st) *st = s->st;
+    return s->st.converged ? 1 : 0;
+}
+
+int egs_solver_stats(egs_solver *s, egs_stats *st) {
+    if (!s || !st) return fail(EGS_E_ARG, "null argument");
+    *st = s->st;
+    return EGS_OK;
+}
+
+int egs_solver_read_values(egs_solver *s, int32_t *host_values) {
+    if (!s || !s->loaded || !host_values) return fail(EGS_E_STATE, "egs_solver_read_values: nothing to read");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemcpyAsync(host_values, s->d_val, sizeof(int) * (size_t)s->g.n, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return EGS_OK;
+}
+
+const int32_t *egs_solver_device_values(egs_solver *s) { return s ? s->d_val : nullptr; }
+
+void *egs_solver_stream(egs_solver *s) { return s ? (void *)s->stream : nullptr; }
+
+int egs_solver_read_csc(egs_solver *s, int32_t *cptr, int32_t *cpred) {
+    if (!s || !s->prepared) return fail(EGS_E_STATE, "egs_solver_read_csc: not prepared");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemcpyAsync(cptr, s->d_cptr, sizeof(int) * ((size_t)s->g.n + 1), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(cpred, s->d_cpred, sizeof(int) * (size_t)s->g.m, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return EGS_OK;
+}
+
+int egs_solver_count_violations(egs_solver *s, int64_t *violations) {
+    if (!s || !s->loaded || !violations) return fail(EGS_E_STATE, "egs_solver_count_violations: nothing loaded");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(egs::launch_violations(s->g, (unsigned long long *)s->d_misc, s->stream));
+    unsigned long long *h = (unsigned long long *)(s->h_probe + 24);
+    HIP_TRY(hipMemcpyAsync(h, s->d_misc, sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    *violations = (int64_t)*h;
+    return EGS_OK;
+}
+
+int egs_solve(int device, int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *ptr,
+              const int32_t *succ, const int32_t *w, int32_t *values, const egs_options *opt,
+              egs_stats *st) {
+    egs_solver *s = nullptr;
+    int rc = egs_solver_create(device, n, m, &s);
+    if (rc) return rc;
+    rc = egs_solver_load(s, EGS_MEM_HOST, n, n0, m, mg, ptr, succ, w);
+    if (!rc) rc = egs_solver_prepare(s, opt);
+    if (!rc) {
+        rc = egs_solver_iterate(s, -1, st);
+        if (rc >= 0) rc = egs_solver_read_values(s, values) ? EGS_E_HIP : EGS_OK;
+    }
+    egs_solver_destroy(s);
+    return rc;
+}
+
+}  // extern "C"

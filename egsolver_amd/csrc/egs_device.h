@@ -1,0 +1,79 @@
+// egs_device.h -- device-side building blocks shared by the single-game, batched and
+// serving kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define EGS_TOP 2147483647
+
+// OMINUS(a, w) under threshold mg (src/common.h:270-272): (a<TOP && a-w<=mg) ?
+// max(0, a-w) : TOP.  The a<TOP test comes first so a-w never overflows.
+__device__ __forceinline__ int egs_ominus(int a, int w, int mg) {
+    const int d = a - w;
+    return (a < EGS_TOP && d <= mg) ? (d > 0 ? d : 0) : EGS_TOP;
+}
+
+// Owner 0 takes the min over out-edges, owner 1 the max (src/cpu_solver.c:300-305).
+// Both are folded into one min-reduction over a key: owner-1 candidates are negated
+// (candidates are >= 0, so -x never overflows).  EGS_TOP is the identity.
+__device__ __forceinline__ int egs_key(int x, bool owner1) { return owner1 ? -x : x; }
+__device__ __forceinline__ int egs_unkey(int k, bool owner1) { return owner1 ? -k : k; }
+
+// ---- DPP / swizzle reductions ----------------------------------------------------------
+// DPP control words (GCN3+/CDNA): quad_perm(1,0,3,2) = 0xB1, quad_perm(2,3,0,1) = 0x4E,
+// row_half_mirror = 0x141, row_mirror = 0x140.  After the steps for group size G every
+// lane of an aligned group of G lanes holds the group minimum.  Callers keep the whole
+// wave active around these calls.
+template <int CTRL>
+__device__ __forceinline__ int egs_dpp(int x) {
+    return __builtin_amdgcn_update_dpp(EGS_TOP, x, CTRL, 0xF, 0xF, false);
+}
+
+template <int G>
+__device__ __forceinline__ int egs_group_min(int x) {
+    if constexpr (G >= 2) x = min(x, egs_dpp<0xB1>(x));
+    if constexpr (G >= 4) x = min(x, egs_dpp<0x4E>(x));
+    if constexpr (G >= 8) x = min(x, egs_dpp<0x141>(x));
+    if constexpr (G >= 16) x = min(x, egs_dpp<0x140>(x));
+    if constexpr (G >= 32) {
+        // ds_swizzle bit-mask mode: and=0x1F, or=0, xor=0x10 -> lane ^ 16 inside each 32.
+        x = min(x, __builtin_amdgcn_ds_swizzle(x, 0x401F));
+    }
+    if constexpr (G >= 64) {
+        x = min(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 32));
+    }
+    return x;
+}
+
+__device__ __forceinline__ int egs_wave_sum(int x) {
+    // Plain butterfly over the wave through ds_swizzle/readlane; used off the hot path.
+    x += __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false);
+    x += __builtin_amdgcn_ds_swizzle(x, 0x401F);
+    return __builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 32);
+}
+
+__device__ __forceinline__ unsigned egs_lane() { return __lane_id(); }
+
+// Lanes below this one in the wave, as a mask.
+__device__ __forceinline__ uint64_t egs_lanemask_lt() {
+    const unsigned l = egs_lane();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Wave-aggregated append: every active lane with want != 0 gets a distinct slot of
+// list[0 .. *count) through one atomic per wave.  Safe under divergence: only active
+// lanes take part.
+__device__ __forceinline__ void egs_append(int *list, int *count, bool want, int item) {
+    const uint64_t mask = __ballot(want);
+    if (mask == 0) return;
+    const uint64_t active = __builtin_amdgcn_read_exec();
+    const int leader = __ffsll((unsigned long long)active) - 1;
+    int base = 0;
+    if ((int)egs_lane() == leader) base = atomicAdd(count, __popcll(mask));
+    base = __builtin_amdgcn_readfirstlane(base);
+    if (want) list[base + __popcll(mask & egs_lanemask_lt())] = item;
+}

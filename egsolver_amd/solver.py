@@ -1,0 +1,113 @@
+"""Single-game solver: Python mirror of the reference's GPU entry (gpu_solver(),
+src/dev_solver.cu:81-139) over libegs.so.
+
+``Solver`` keeps one game resident on the device; ``solve()`` is the one-shot form.
+The values it returns are the least fixpoint of the energy lifting, bit-identical to the
+reference's cpu_solver.c on the same game.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+from .games import Game
+
+
+def options(group=0, big_deg=0, huge_deg=0, grid=0, batch_rounds=0, max_phases=-1, stop_flag=None):
+    o = N.EgsOptions(group=group, big_deg=big_deg, huge_deg=huge_deg, grid=grid,
+                     batch_rounds=batch_rounds, reserved0=0, max_phases=max_phases, stop_flag=None)
+    if stop_flag is not None:
+        o.stop_flag = ctypes.cast(ctypes.pointer(stop_flag), ctypes.c_void_p)
+    return o
+
+
+class Solver:
+    """One device workspace sized for games up to (n_cap, m_cap)."""
+
+    def __init__(self, n_cap, m_cap, device=0):
+        h = ctypes.c_void_p()
+        N.check(N.egs().egs_solver_create(device, int(n_cap), int(m_cap), ctypes.byref(h)), "egs_solver_create")
+        self.h = h
+        self.n = 0
+        self.m = 0
+
+    def close(self):
+        if self.h:
+            N.egs().egs_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def load(self, g: Game):
+        ptr = np.ascontiguousarray(g.ptr, np.int32)
+        succ = np.ascontiguousarray(g.succ, np.int32)
+        w = np.ascontiguousarray(g.w, np.int32)
+        N.check(N.egs().egs_solver_load(self.h, 0, g.n, g.n0, g.m, g.mg, ptr.ctypes.data, succ.ctypes.data,
+                                        w.ctypes.data), "egs_solver_load")
+        self.n, self.m = g.n, g.m
+
+    def load_device(self, n, n0, m, mg, d_ptr, d_succ, d_w):
+        """Arrays already resident in device memory (integer device addresses)."""
+        N.check(N.egs().egs_solver_load(self.h, 1, n, n0, m, mg, d_ptr, d_succ, d_w), "egs_solver_load")
+        self.n, self.m = n, m
+
+    def prepare(self, opt=None):
+        opt = opt or options()
+        N.check(N.egs().egs_solver_prepare(self.h, ctypes.byref(opt)), "egs_solver_prepare")
+
+    def iterate(self, max_phases=-1):
+        st = N.EgsStats()
+        rc = N.check(N.egs().egs_solver_iterate(self.h, int(max_phases), ctypes.byref(st)), "egs_solver_iterate")
+        return bool(rc), st
+
+    def values(self):
+        out = np.empty(self.n, np.int32)
+        N.check(N.egs().egs_solver_read_values(self.h, out.ctypes.data), "egs_solver_read_values")
+        return out
+
+    def csc(self):
+        cptr = np.empty(self.n + 1, np.int32)
+        cpred = np.empty(self.m, np.int32)
+        N.check(N.egs().egs_solver_read_csc(self.h, cptr.ctypes.data, cpred.ctypes.data), "egs_solver_read_csc")
+        return cptr, cpred
+
+    def violations(self):
+        v = ctypes.c_int64(0)
+        N.check(N.egs().egs_solver_count_violations(self.h, ctypes.byref(v)), "egs_solver_count_violations")
+        return int(v.value)
+
+    def stats(self):
+        st = N.EgsStats()
+        N.check(N.egs().egs_solver_stats(self.h, ctypes.byref(st)), "egs_solver_stats")
+        return st
+
+
+def solve(g: Game, device=0, **opt):
+    """Solve one game to its fixpoint; returns (values int32[n], stats dict)."""
+    values = np.empty(g.n, np.int32)
+    st = N.EgsStats()
+    o = options(**opt)
+    ptr = np.ascontiguousarray(g.ptr, np.int32)
+    succ = np.ascontiguousarray(g.succ, np.int32)
+    w = np.ascontiguousarray(g.w, np.int32)
+    N.check(N.egs().egs_solve(device, g.n, g.n0, g.m, g.mg, ptr.ctypes.data, succ.ctypes.data, w.ctypes.data,
+                              values.ctypes.data, ctypes.byref(o), ctypes.byref(st)), "egs_solve")
+    return values, st.as_dict()
+
+
+def device_info(device=0):
+    info = N.EgsDeviceInfo()
+    N.check(N.egs().egs_device_query(device, ctypes.byref(info)), "egs_device_query")
+    return info

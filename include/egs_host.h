@@ -1,0 +1,113 @@
+/*
+ * egs_host.h -- host-side C library of the MI355X energy-game solver (libegs_host.so).
+ *
+ * Input side of the solver path: the .mpg reader, the owner partition into the internal
+ * numbering, the MG threshold, and this project's seeded game generator.  Plain C, no
+ * GPU calls.  The reader replaces the reference's flex/bison front end
+ * (src/the_lex.l:33-52, src/the_parser.y:63-102,128-295) and remap/postparsing
+ * (src/egsolver.c:107-315).
+ */
+#ifndef EGS_HOST_H
+#define EGS_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- parsed game in read order ------------------------------------------------------ */
+
+/* Weight rewriting modes applied while reading (src/the_parser.y:324-327,
+ * options --unit-weights / --rnd-weights / --exp-weights / --no-weights). */
+enum { EGS_W_KEEP = 0, EGS_W_UNIT = 1, EGS_W_EXP = 2, EGS_W_RND = 3 };
+
+typedef struct egs_mpg {
+    int32_t n;            /* vertices: max id + 1 */
+    int64_t m;            /* edges */
+    int32_t *ext_of_read; /* external id of the i-th vertex read */
+    int32_t *read_of_ext; /* inverse, -1 where unused */
+    uint8_t *owner;       /* owner of the i-th vertex read: 0 = MAX token, 1 = MIN token */
+    int32_t *rptr;        /* n+1 offsets into rsucc/rw (read order) */
+    int32_t *rsucc;       /* successor EXTERNAL ids */
+    int32_t *rw;          /* weights after the rewrite mode */
+    int32_t max_abs_w;    /* max |w| (src/the_parser.y:330) */
+    int32_t min_outdeg, max_outdeg;
+    int32_t low_outdeg[4];
+    char **names;         /* optional "string" per vertex read, or NULL entries */
+} egs_mpg;
+
+/* Parse a .mpg file (plain or gzip, path NULL = stdin).  Weight mode and the
+ * rnd interval follow the reference; EGS_W_RND draws rand() per edge in read order
+ * after the caller's srand().  Returns 0, or -1 with a message in errbuf (the reference
+ * prints the same facts and exits). */
+int egs_mpg_read(const char *path, int weight_mode, int rnd_lo, int rnd_hi, egs_mpg *out,
+                 char *errbuf, int errlen);
+int egs_mpg_read_mem(const char *text, int64_t len, int weight_mode, int rnd_lo, int rnd_hi,
+                     egs_mpg *out, char *errbuf, int errlen);
+void egs_mpg_free(egs_mpg *g);
+
+/* Internal numbering of a parsed game: owner-0 vertices first, both blocks in read order
+ * (src/egsolver.c:146-214 with the default comparator, src/sorting_criteria.h:91-102),
+ * then the internal CSR (src/egsolver.c:253-281).  mapping[i] = read index of internal
+ * vertex i.  Returns n0 (the reference's counter_nodi0). */
+int egs_mpg_internalise(const egs_mpg *g, int32_t *mapping, int32_t *ptr, int32_t *succ,
+                        int32_t *w);
+
+/* Same partition for arrays whose read order equals external ids (generator output). */
+int egs_internalise(int n, const uint8_t *owner, const int32_t *rptr, const int32_t *rsucc,
+                    const int32_t *rw, int32_t *ptr, int32_t *succ, int32_t *w,
+                    int32_t *ext_of_int);
+
+/* MG = sum_v max(0, max_e -w_e) (src/egsolver.c:283-304).  -1 on int32 overflow. */
+int egs_compute_mg(int n, const int32_t *ptr, const int32_t *w, int32_t *mg_out);
+
+/* Loop bound of src/egsolver.c:64-90: maxloop < 0 means n*m*MG+1, saturating at
+ * LONG_MAX-1. */
+long egs_max_loop(long m, long n, long mg, long maxloop);
+
+/* ---- generator ---------------------------------------------------------------------- */
+
+enum { EGS_DEG_FIXED = 0, EGS_DEG_UNIFORM = 1, EGS_DEG_POWERLAW = 2, EGS_DEG_COMPLETE = 3 };
+
+typedef struct egs_gen_params {
+    uint64_t seed;
+    int32_t n;
+    int32_t deg_kind;     /* EGS_DEG_* */
+    int32_t deg_lo, deg_hi;
+    double alpha;         /* power-law exponent */
+    int32_t wmax;         /* weights uniform in [-wmax, wmax] */
+    int32_t reserved;
+} egs_gen_params;
+
+typedef struct egs_gen_read_order {
+    uint8_t *owner;       /* n */
+    int32_t *ptr;         /* n+1 */
+    int32_t *succ;        /* m */
+    int32_t *w;           /* m */
+} egs_gen_read_order;
+
+/* Generate game `stream` of the family: arrays in external-id order.  With count_only
+ * the arrays are untouched and only the edge count is returned.  Returns m. */
+int64_t egs_gen_game(const egs_gen_params *p, uint64_t stream, egs_gen_read_order *out,
+                     int count_only);
+int64_t egs_gen_count_edges(const egs_gen_params *p, uint64_t stream);
+
+/* Instance parameters of the multi-game families: index i of family `family`
+ * (EGS_FAMILY_BATCH: n log-uniform in [1e3, 5e4]; EGS_FAMILY_SERVE: [1e3, 5e5]);
+ * out-degree uniform in [2,16]; W log-uniform in [10,1000]; every 16th instance
+ * (i % 16 == 15) a complete digraph on 200..500 vertices. */
+enum { EGS_FAMILY_BATCH = 4, EGS_FAMILY_SERVE = 5 };
+void egs_gen_family_params(int family, uint64_t seed, int index, egs_gen_params *out);
+
+/* Raw PRNG output, for the generator's own known-answer test. */
+uint64_t egs_rng_test_stream(uint64_t seed, uint64_t stream, int count, uint64_t *out);
+
+/* Write a game in .mpg format (read order, external ids 0..n-1); gz != 0 for gzip. */
+int egs_write_mpg(const char *path, int n, const uint8_t *owner, const int32_t *rptr,
+                  const int32_t *rsucc, const int32_t *rw, int gz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EGS_HOST_H */

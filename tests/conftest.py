@@ -1,0 +1,35 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle.oracle import Oracle
+    return Oracle()
+
+
+def load_golden_sol(name="clique_500_cp4.sol"):
+    vals = {}
+    with open(os.path.join(GOLDEN, name)) as f:
+        for line in f:
+            if line.startswith("#"):
+                continue
+            e, v = line.split()
+            vals[int(e)] = int(v)
+    return vals
+
+
+def to_oracle_game(g):
+    from oracle.oracle import game_from_arrays
+    return game_from_arrays(g.n, g.n0, g.ptr, g.succ, g.w, g.ext_of_int)
