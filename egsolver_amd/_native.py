@@ -37,7 +37,7 @@ class EgsOptions(ctypes.Structure):
         ("huge_deg", ctypes.c_int32),
         ("grid", ctypes.c_int32),
         ("batch_rounds", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("dense_pct", ctypes.c_int32),
         ("max_phases", ctypes.c_int64),
         ("stop_flag", ctypes.c_void_p),
     ]
@@ -74,6 +74,8 @@ class EgsDeviceInfo(ctypes.Structure):
         ("ecc", ctypes.c_int32),
         ("driver_version", ctypes.c_int32),
         ("runtime_version", ctypes.c_int32),
+        ("cap_major", ctypes.c_int32),
+        ("cap_minor", ctypes.c_int32),
     ]
 
 
@@ -132,6 +134,10 @@ EGS_SIGNATURES = {
     "egs_solve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                  _vp, _vp, _vp, _vp, ctypes.POINTER(EgsOptions), ctypes.POINTER(EgsStats)]),
     "egs_solver_count_violations": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "egs_host_alloc": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_size_t]),
+    "egs_host_free": (ctypes.c_int, [_vp]),
+    "egs_runtime_error": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    "egs_copy_bandwidth": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
 }
 
 EGS_HOST_SIGNATURES = {

@@ -54,6 +54,15 @@ int64_t egs_max_loop_c(int64_t m, int64_t n, int64_t mg, int64_t maxloop) {
     return n * m * mg + 1;
 }
 
+// Offsets (in ints) inside the solver's pinned readback block.
+constexpr int kPinProbe = 0;      // 2 x 128: per-batch counter probes
+constexpr int kPinDeg = 256;
+constexpr int kPinInit = 320;     // kListCounters ints
+constexpr int kPinConv = 448;     // kListCounters ints
+constexpr int kPinRounds = 576;
+constexpr int kPinStats = 640;    // 4 x u64
+constexpr int kPinViol = 704;     // 1 x u64
+
 int pow2ceil(int x) {
     int p = 1;
     while (p < x) p <<= 1;
@@ -70,8 +79,10 @@ struct egs_solver {
     void *block = nullptr;
     size_t block_bytes = 0;
     int *d_ptr = nullptr, *d_cptr = nullptr, *d_cpred = nullptr, *d_val = nullptr;
-    int *d_stamp = nullptr, *d_q[2] = {nullptr, nullptr}, *d_big = nullptr, *d_huge = nullptr;
+    int *d_stamp = nullptr, *d_list[2] = {nullptr, nullptr}, *d_dbig = nullptr, *d_dhuge = nullptr;
+    int *d_sbig = nullptr, *d_shuge = nullptr, *d_lcnt = nullptr, *d_mode = nullptr;
     int *d_ring = nullptr, *d_rounds = nullptr, *d_misc = nullptr, *d_tile = nullptr, *d_fill = nullptr;
+    int sub_cap = 0;
     int2 *d_edges = nullptr;
     unsigned long long *d_stats = nullptr, *d_statsum = nullptr;
     int *h_probe = nullptr;   // pinned: per-batch counter probes + misc readbacks
@@ -88,6 +99,13 @@ struct egs_solver {
     const volatile uint32_t *stop_flag = nullptr;
     egs_stats st{};
 };
+
+// Items in a list from its counters (sub-queues clamp at sub_cap; the rest overflowed).
+static int64_t list_total(const egs_solver *s, const int *c) {
+    int64_t t = 0;
+    for (int k = 0; k < egs::kSub; k++) t += std::min(c[k], s->sub_cap);
+    return t + c[egs::kSub];
+}
 
 extern "C" {
 
@@ -117,7 +135,28 @@ int egs_device_query(int device, egs_device_info *out) {
     HIP_TRY(hipRuntimeGetVersion(&rv));
     out->driver_version = dv;
     out->runtime_version = rv;
+    out->cap_major = p.major;
+    out->cap_minor = p.minor;
     return EGS_OK;
+}
+
+int egs_host_alloc(void **p, size_t bytes) {
+    if (!p) return fail(EGS_E_ARG, "egs_host_alloc: null output");
+    *p = nullptr;
+    if (hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess)
+        return fail(EGS_E_NOMEM, "hipHostMalloc of %zu bytes failed", bytes);
+    return EGS_OK;
+}
+
+int egs_host_free(void *p) {
+    if (p) HIP_TRY(hipHostFree(p));
+    return EGS_OK;
+}
+
+int egs_runtime_error(char *buf, int len) {
+    hipError_t e = hipGetLastError();
+    if (buf && len > 0) snprintf(buf, (size_t)len, "%s", e == hipSuccess ? "" : hipGetErrorString(e));
+    return (int)e;
 }
 
 int egs_device_reset(int device) {
@@ -155,11 +194,15 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
     s->m_cap = m_cap;
     const size_t n1 = (size_t)n_cap + 1;
     const int tiles = (n_cap + 4095) / 4096 + 1;
+    s->sub_cap = std::max(256, (int)((2 * (int64_t)n_cap + egs::kSub - 1) / egs::kSub));
+    const size_t list_ints = (size_t)egs::kSub * s->sub_cap + n1;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
     size_t o_ptr = take(4 * n1), o_edges = take(8 * (size_t)m_cap), o_cptr = take(4 * n1),
            o_cpred = take(4 * (size_t)m_cap), o_val = take(4 * n1), o_stamp = take(4 * n1),
-           o_q0 = take(4 * n1), o_q1 = take(4 * n1), o_big = take(4 * n1), o_huge = take(4 * n1),
+           o_l0 = take(4 * list_ints), o_l1 = take(4 * list_ints), o_dbig = take(4 * n1),
+           o_dhuge = take(4 * n1), o_sbig = take(4 * n1), o_shuge = take(4 * n1),
+           o_lcnt = take(4 * 3 * egs::kListCounters), o_mode = take(4 * 4),
            o_ring = take(4 * 16), o_rounds = take(4 * 16), o_misc = take(8 * 16),
            o_tile = take(4 * (size_t)tiles), o_fill = take(4 * n1),
            o_stats = take(8 * 4 * (size_t)egs::kStatSlots), o_statsum = take(8 * 8);
@@ -175,10 +218,14 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
     s->d_cpred = (int *)(b + o_cpred);
     s->d_val = (int *)(b + o_val);
     s->d_stamp = (int *)(b + o_stamp);
-    s->d_q[0] = (int *)(b + o_q0);
-    s->d_q[1] = (int *)(b + o_q1);
-    s->d_big = (int *)(b + o_big);
-    s->d_huge = (int *)(b + o_huge);
+    s->d_list[0] = (int *)(b + o_l0);
+    s->d_list[1] = (int *)(b + o_l1);
+    s->d_dbig = (int *)(b + o_dbig);
+    s->d_dhuge = (int *)(b + o_dhuge);
+    s->d_sbig = (int *)(b + o_sbig);
+    s->d_shuge = (int *)(b + o_shuge);
+    s->d_lcnt = (int *)(b + o_lcnt);
+    s->d_mode = (int *)(b + o_mode);
     s->d_ring = (int *)(b + o_ring);
     s->d_rounds = (int *)(b + o_rounds);
     s->d_misc = (int *)(b + o_misc);
@@ -186,7 +233,7 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
     s->d_fill = (int *)(b + o_fill);
     s->d_stats = (unsigned long long *)(b + o_stats);
     s->d_statsum = (unsigned long long *)(b + o_statsum);
-    if (hipHostMalloc((void **)&s->h_probe, 256, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&s->h_probe, 8192, hipHostMallocDefault) != hipSuccess) {
         egs_solver_destroy(s);
         return fail(EGS_E_NOMEM, "hipHostMalloc failed");
     }
@@ -236,9 +283,9 @@ int egs_solver_load(egs_solver *s, int mem, int32_t n, int32_t n0, int64_t m, in
         HIP_TRY(hipMemcpyAsync(s->d_ptr, ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToDevice, s->stream));
         HIP_TRY(egs::launch_pack(succ, w, s->d_edges, m, s->stream));
         HIP_TRY(egs::launch_degree_stats(s->d_ptr, n, s->d_misc, s->stream));
-        HIP_TRY(hipMemcpyAsync(s->h_probe + 8, s->d_misc, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->h_probe + kPinDeg, s->d_misc, sizeof(int), hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
-        s->max_outdeg = s->h_probe[8];
+        s->max_outdeg = s->h_probe[kPinDeg];
     } else {
         return fail(EGS_E_ARG, "mem must be EGS_MEM_HOST or EGS_MEM_DEVICE");
     }
@@ -253,10 +300,15 @@ int egs_solver_load(egs_solver *s, int mem, int32_t n, int32_t n0, int64_t m, in
     g.cpred = s->d_cpred;
     g.val = s->d_val;
     g.stamp = s->d_stamp;
-    g.queue[0] = s->d_q[0];
-    g.queue[1] = s->d_q[1];
-    g.big = s->d_big;
-    g.huge = s->d_huge;
+    g.list[0] = s->d_list[0];
+    g.list[1] = s->d_list[1];
+    g.sub_cap = s->sub_cap;
+    g.lcnt = s->d_lcnt;
+    g.mode = s->d_mode;
+    g.dbig = s->d_dbig;
+    g.dhuge = s->d_dhuge;
+    g.sbig = s->d_sbig;
+    g.shuge = s->d_shuge;
     g.ring = s->d_ring;
     g.stats = s->d_stats;
     g.rounds_done = s->d_rounds;
@@ -266,7 +318,7 @@ int egs_solver_load(egs_solver *s, int mem, int32_t n, int32_t n0, int64_t m, in
 }
 
 static int pick_shape(egs_solver *s, const egs_options *o) {
-    const egs::DevGame &g = s->g;
+    egs::DevGame &g = s->g;
     egs::LaunchShape &sh = s->sh;
     egs_options def{};
     if (!o) o = &def;
@@ -276,13 +328,15 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     if (G < 1 || G > 64 || (G & (G - 1)))
         return fail(EGS_E_ARG, "group must be a power of two in 1..64 (got %d)", G);
     sh.group = G;
-    sh.big_deg = o->big_deg > 0 ? o->big_deg : std::max(64, 8 * G);
-    sh.huge_deg = o->huge_deg > 0 ? o->huge_deg : 4096;
-    if (sh.huge_deg < sh.big_deg) sh.huge_deg = sh.big_deg;
-    sh.grid = o->grid > 0 ? o->grid : 2048;
+    g.big_deg = o->big_deg > 0 ? o->big_deg : std::max(64, 8 * G);
+    g.huge_deg = o->huge_deg > 0 ? o->huge_deg : 4096;
+    if (g.huge_deg < g.big_deg) g.huge_deg = g.big_deg;
+    const int pct = o->dense_pct > 0 ? o->dense_pct : 12;
+    g.dense_min = pct > 100 ? INT64_MAX : std::max<int64_t>(1, (int64_t)g.n * pct / 100);
+    sh.grid = o->grid > 0 ? o->grid : 1024;
     if (sh.grid > egs::kStatSlots) sh.grid = egs::kStatSlots;
-    sh.use_wave = s->max_outdeg > sh.big_deg;
-    sh.use_block = s->max_outdeg > sh.huge_deg;
+    sh.use_wave = s->max_outdeg > g.big_deg;
+    sh.use_block = s->max_outdeg > g.huge_deg;
     sh.wave_grid = 1024;
     sh.block_grid = 512;
     s->batch_rounds = o->batch_rounds > 0 ? o->batch_rounds : 32;
@@ -300,15 +354,17 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     HIP_TRY(hipMemsetAsync(s->d_stats, 0, sizeof(unsigned long long) * 4 * egs::kStatSlots, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_rounds, 0, sizeof(int), s->stream));
     HIP_TRY(egs::launch_transpose(s->g, s->d_tile, s->d_fill, s->stream));
+    HIP_TRY(egs::launch_classify(s->g, s->d_ring + 9, s->stream));
     HIP_TRY(egs::launch_init(s->g, s->stream));
     HIP_TRY(hipEventRecord(s->ev_b, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->h_probe + 9, s->d_ring + 1, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_probe + kPinInit, s->d_lcnt + 1 * egs::kListCounters,
+                           sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, s->ev_a, s->ev_b));
     memset(&s->st, 0, sizeof(s->st));
     s->st.ms_prepare = ms;
-    s->st.initial_active = s->h_probe[9];
+    s->st.initial_active = (int32_t)list_total(s, s->h_probe + kPinInit);
     s->next_round = 1;
     s->prepared = true;
     return EGS_OK;
@@ -316,18 +372,18 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
 
 static int collect_stats(egs_solver *s) {
     HIP_TRY(egs::launch_sum_stats(s->g, egs::kStatSlots, s->d_statsum, s->stream));
-    unsigned long long *h = (unsigned long long *)(s->h_probe + 16);   // 8-byte aligned
+    unsigned long long *h = (unsigned long long *)(s->h_probe + kPinStats);
     HIP_TRY(hipMemcpyAsync(h, s->d_statsum, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->h_probe + 10, s->d_rounds, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->h_probe + 11, s->d_ring + (s->next_round % 3), sizeof(int),
-                           hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_probe + kPinRounds, s->d_rounds, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_probe + kPinConv, s->d_lcnt + (s->next_round % 3) * egs::kListCounters,
+                           sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->st.processed = (int64_t)h[0];
     s->st.edges_relaxed = (int64_t)h[1];
     s->st.lifted = (int64_t)h[2];
     s->st.pred_visits = (int64_t)h[3];
-    s->st.phases = s->h_probe[10];
-    s->st.converged = s->h_probe[11] == 0;
+    s->st.phases = s->h_probe[kPinRounds];
+    s->st.converged = list_total(s, s->h_probe + kPinConv) == 0;
     return EGS_OK;
 }
 
@@ -350,12 +406,13 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
         }
         launched += nb;
         const int slot = (pending + 1) & 1;
-        HIP_TRY(hipMemcpyAsync(s->h_probe + slot, s->d_ring + (s->next_round % 3), sizeof(int),
-                               hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->h_probe + kPinProbe + slot * 128,
+                               s->d_lcnt + (s->next_round % 3) * egs::kListCounters,
+                               sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipEventRecord(s->ev_probe[slot], s->stream));
         if (pending >= 0) {
             HIP_TRY(hipEventSynchronize(s->ev_probe[pending]));
-            if (s->h_probe[pending] == 0) done = true;
+            if (list_total(s, s->h_probe + kPinProbe + pending * 128) == 0) done = true;
         }
         pending = slot;
         if (s->stop_flag && *s->stop_flag) break;
@@ -402,7 +459,7 @@ int egs_solver_count_violations(egs_solver *s, int64_t *violations) {
     if (!s || !s->loaded || !violations) return fail(EGS_E_STATE, "egs_solver_count_violations: nothing loaded");
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(egs::launch_violations(s->g, (unsigned long long *)s->d_misc, s->stream));
-    unsigned long long *h = (unsigned long long *)(s->h_probe + 24);
+    unsigned long long *h = (unsigned long long *)(s->h_probe + kPinViol);
     HIP_TRY(hipMemcpyAsync(h, s->d_misc, sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     *violations = (int64_t)*h;

@@ -6,7 +6,19 @@
 
 namespace egs {
 
-// One game resident on the device, passed by value to every round kernel.
+// Frontier lists are split into kSub sub-queues of sub_cap slots plus one overflow region
+// of n slots; a workgroup appends to sub-queue (blockIdx.x % kSub), so no counter sees
+// more than a few dozen workgroups' atomics per round.
+constexpr int kSub = 64;
+constexpr int kListCounters = kSub + 1;     // kSub sub-queue counters + overflow counter
+constexpr int kLdsCap = 2048;               // per-workgroup LDS staging of appends
+constexpr int kStatSlots = 8192;            // upper bound on any round kernel's grid
+
+// Round modes, decided on the device at the start of every round from the previous
+// round's mode and output-list size (see decide_mode in egs_rounds.hip).
+enum : int { MODE_QUEUE = 0, MODE_LIFTED = 1, MODE_DENSE = 2, MODE_DONE = 3 };
+
+// One game resident on the device, passed by value to every kernel.
 struct DevGame {
     int n, n0, mg;
     int64_t m;
@@ -15,33 +27,37 @@ struct DevGame {
     const int *cptr;      // n+1 CSC offsets
     const int *cpred;     // m predecessors
     int *val;             // n values
-    int *stamp;           // n: last round a vertex was queued for
-    int *queue[2];        // frontier ping-pong, n each
-    int *big;             // n: vertices deferred to the wavefront kernel this round
-    int *huge;            // n: vertices deferred to the workgroup kernel this round
-    int *ring;            // 9 counters: [0..2] queue, [3..5] big, [6..8] huge
+    int *stamp;           // n: last round a vertex was claimed for / queued for
+    int *list[2];         // output lists, ping-pong: kSub*sub_cap + n slots each
+    int sub_cap;
+    int *lcnt;            // 3 slots x kListCounters
+    int *mode;            // 3 slots: mode of round r at r % 3
+    int *dbig, *dhuge;    // per-round deferral lists (sparse modes)
+    int *ring;            // [3..5] dbig counters, [6..8] dhuge counters, [9] nsbig, [10] nshuge
+    const int *sbig;      // static: vertices with big_deg < deg <= huge_deg
+    const int *shuge;     // static: vertices with deg > huge_deg
+    int big_deg, huge_deg;
+    int64_t dense_min;    // estimated work (vertices) at or above which a round sweeps all
     unsigned long long *stats;  // per-workgroup slots of {processed, edges, lifted, preds}
-    int *rounds_done;     // last round that found a non-empty frontier
+    int *rounds_done;     // last round that was not MODE_DONE
 };
 
 // Launch shape of one round.
 struct LaunchShape {
-    int group;       // lanes per vertex in k_lift_groups (1..64)
-    int grid;        // workgroups of k_lift_groups
-    int big_deg;     // out-degree above which a vertex goes to k_lift_wave
-    int huge_deg;    // out-degree above which a vertex goes to k_lift_block
+    int group;       // lanes per vertex in k_round (1..64)
+    int grid;        // workgroups of k_round
     int use_wave, use_block;
     int wave_grid, block_grid;
 };
 
-constexpr int kStatSlots = 8192;   // upper bound on any round kernel's grid
-
 hipError_t launch_pack(const int *succ, const int *w, int2 *edges, int64_t m, hipStream_t st);
 hipError_t launch_transpose(const DevGame &g, int *tile_sum, int *fill, hipStream_t st);
+hipError_t launch_classify(const DevGame &g, int *counts, hipStream_t st);
 hipError_t launch_init(const DevGame &g, hipStream_t st);
 hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int r, hipStream_t st);
 hipError_t launch_sum_stats(const DevGame &g, int nslots, unsigned long long *out, hipStream_t st);
-hipError_t launch_degree_stats(const int *ptr, int n, int *out2, hipStream_t st);
+hipError_t launch_degree_stats(const int *ptr, int n, int *out, hipStream_t st);
 hipError_t launch_violations(const DevGame &g, unsigned long long *out, hipStream_t st);
+int grid_for(int64_t work, int threads, int cap);
 
 }  // namespace egs

@@ -1,0 +1,515 @@
+// egs_rounds.hip -- initial frontier and the fixpoint rounds of one resident game.
+//
+// Replaces kernel_EG_initialize (src/dev_EG_alg.cu:176-205), the lift kernels with 1..32
+// lanes per vertex (src/dev_EG_alg.cu:73-162, 397-1480) and the ping-pong round driver
+// with its thrust compaction and memsets (src/dev_EG_alg.cu:307-381,
+// src/thrust_wrapper.cu:30-41).
+//
+// Every round r runs one of three bodies, chosen on the device from round r-1's output:
+//   DENSE   sweep all vertices in id order (edges stream, values update in place) and
+//           list the vertices that rose.  Used while most vertices are active.
+//   QUEUE   process the worklist built by round r-1; every vertex that rises queues its
+//           predecessors for round r+1 (deduplicated by stamp[u] = r+1).
+//   LIFTED  after a dense round: claim each predecessor of the vertices listed as risen
+//           (stamp[p] = r) and process it; risers queue their predecessors as in QUEUE.
+// All bodies read neighbour values in place.  Values only rise and never pass the least
+// fixpoint, and any vertex with a successor that rose during round r is in the work of
+// round r+1 (every vertex for DENSE; its predecessors for QUEUE/LIFTED), so an empty
+// output ends the run exactly at lift(val) == val.
+//
+// Vertices of out-degree above big_deg are taken out of the main kernel: a wavefront per
+// vertex (k_lift_wave) and, above huge_deg, a workgroup per vertex (k_lift_block), in the
+// same round.  Appends are staged in LDS per workgroup and flushed with one atomic per
+// sub-queue counter.
+#include "egs_device.h"
+#include "egs_internal.h"
+
+namespace egs {
+
+// ---------------------------------------------------------------------------------------
+// list helpers
+// ---------------------------------------------------------------------------------------
+
+// Reserve k slots of an output list for sub-queue `sub`: the first n_sub go to the
+// sub-queue at base_sub, the rest to the overflow region at base_ovf.
+__device__ __forceinline__ void reserve_split(const DevGame &g, int *cnts, int sub, int k,
+                                              int &base_sub, int &n_sub, int &base_ovf) {
+    const int p = atomicAdd(&cnts[sub], k);
+    const int room = max(0, g.sub_cap - p);
+    n_sub = min(k, room);
+    base_sub = p;
+    base_ovf = (k > n_sub) ? atomicAdd(&cnts[kSub], k - n_sub) : 0;
+}
+
+__device__ __forceinline__ int list_slot(const DevGame &g, int sub, int base_sub, int n_sub,
+                                         int base_ovf, int i) {
+    return i < n_sub ? sub * g.sub_cap + base_sub + i : kSub * g.sub_cap + base_ovf + (i - n_sub);
+}
+
+// Wave-aggregated append straight to the output list (LDS spill and rare appends).
+__device__ __forceinline__ void push_wave(const DevGame &g, int *list, int *cnts, int sub, bool want,
+                                          int item) {
+    const uint64_t mask = __ballot(want);
+    if (mask == 0) return;
+    const int leader = __ffsll((unsigned long long)__builtin_amdgcn_read_exec()) - 1;
+    int bs = 0, ns = 0, bo = 0;
+    if ((int)egs_lane() == leader) reserve_split(g, cnts, sub, __popcll(mask), bs, ns, bo);
+    bs = __builtin_amdgcn_readfirstlane(bs);
+    ns = __builtin_amdgcn_readfirstlane(ns);
+    bo = __builtin_amdgcn_readfirstlane(bo);
+    if (want) list[list_slot(g, sub, bs, ns, bo, __popcll(mask & egs_lanemask_lt()))] = item;
+}
+
+// Per-workgroup LDS staging of appends.
+struct Stage {
+    int *buf;   // kLdsCap ints in LDS
+    int *cnt;   // LDS counter (may pass kLdsCap; the excess went to global directly)
+    int *list;  // output list
+    int *cnts;  // its counters
+    int sub;
+};
+
+__device__ __forceinline__ void stage_push(const DevGame &g, const Stage &s, bool want, int item) {
+    const uint64_t mask = __ballot(want);
+    if (mask == 0) return;
+    const int leader = __ffsll((unsigned long long)__builtin_amdgcn_read_exec()) - 1;
+    int base = 0;
+    if ((int)egs_lane() == leader) base = atomicAdd(s.cnt, __popcll(mask));
+    base = __builtin_amdgcn_readfirstlane(base);
+    const int pos = base + __popcll(mask & egs_lanemask_lt());
+    const bool spill = want && pos >= kLdsCap;
+    if (want && !spill) s.buf[pos] = item;
+    push_wave(g, s.list, s.cnts, s.sub, spill, item);
+}
+
+// Block-uniform: move the staged items to the output list when at least `threshold`.
+__device__ __forceinline__ void stage_flush(const DevGame &g, const Stage &s, int threshold) {
+    __shared__ int fb[3];
+    __syncthreads();
+    const int c = min(*s.cnt, kLdsCap);
+    __syncthreads();    // every thread has read the count before anyone changes it
+    if (c == 0 || c < threshold) return;
+    if (threadIdx.x == 0) reserve_split(g, s.cnts, s.sub, c, fb[0], fb[1], fb[2]);
+    __syncthreads();
+    const int bs = fb[0], ns = fb[1], bo = fb[2];
+    for (int i = threadIdx.x; i < c; i += blockDim.x) s.list[list_slot(g, s.sub, bs, ns, bo, i)] = s.buf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) *s.cnt = 0;
+    __syncthreads();
+}
+
+// Input list layout: pre[k] = items before sub-queue k (k = 0..kSub, the last being the
+// overflow region), pre[kSub+1] = total.  Wave 0 builds it; callers sync.
+__device__ __forceinline__ void load_layout(const DevGame &g, int slot, int *pre) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const int *c = g.lcnt + slot * kListCounters;
+        int x = 0;
+        if (lane < kSub) x = min(c[lane], g.sub_cap);
+        int ovf = c[kSub];
+        int inc = x;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += y;
+        }
+        pre[lane] = inc - x;
+        if (lane == 63) {
+            pre[kSub] = inc;
+            pre[kSub + 1] = inc + ovf;
+        }
+    }
+}
+
+// Position in the list of the i-th item of the layout.
+__device__ __forceinline__ int layout_slot(const DevGame &g, const int *pre, int i) {
+    int lo = 0, hi = kSub;        // largest k in [0, kSub] with pre[k] <= i
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo * g.sub_cap + (i - pre[lo]);
+}
+
+// Round mode from round r-1's mode and output size (identical in every workgroup).
+__device__ __forceinline__ int decide_mode(const DevGame &g, int r, int total) {
+    const int prev = g.mode[(r + 2) % 3];
+    if (prev == MODE_DONE || total == 0) return MODE_DONE;
+    if (prev == MODE_DENSE) {
+        const int64_t est = (int64_t)total * g.m / g.n;   // vertices the risers' predecessors cover
+        return est >= g.dense_min ? MODE_DENSE : MODE_LIFTED;
+    }
+    return total >= g.dense_min ? MODE_DENSE : MODE_QUEUE;
+}
+
+// ---------------------------------------------------------------------------------------
+// statistics
+// ---------------------------------------------------------------------------------------
+
+struct LiftAcc {
+    uint32_t processed = 0, lifted = 0;
+    uint64_t edges = 0, preds = 0;
+};
+
+__device__ __forceinline__ void flush_stats(const DevGame &g, LiftAcc &a) {
+    __shared__ unsigned long long red[4][4];
+    const int wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    unsigned long long vals[4] = {a.processed, a.edges, a.lifted, a.preds};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        unsigned long long x = vals[k];
+        for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+        if (lane == 0) red[k][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        unsigned long long s = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[threadIdx.x][i];
+        if (s) g.stats[(size_t)blockIdx.x * 4 + threadIdx.x] += s;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// pieces of a lift
+// ---------------------------------------------------------------------------------------
+
+// min/max over v's out-edges [s, e) of OMINUS(val[succ], w), lanes of a group of G
+// striding; returns the group result in every lane of the group.
+template <int G>
+__device__ __forceinline__ int lift_value(const DevGame &g, int s, int e, int gl, bool owner1) {
+    int key = EGS_TOP;
+    for (int k = s + gl; k < e; k += G) {
+        const int2 ew = g.edges[k];
+        key = min(key, egs_key(egs_ominus(g.val[ew.x], ew.y, g.mg), owner1));
+    }
+    key = egs_group_min<G>(key);
+    return egs_unkey(key, owner1);
+}
+
+// Queue the predecessors of a risen vertex v for round r+1 (stamp dedupe), lanes of a
+// group of G striding over v's CSC list.
+template <int G>
+__device__ __forceinline__ void push_preds(const DevGame &g, const Stage &st, int v, bool lifted, int gl,
+                                           int r, LiftAcc &acc) {
+    int cs = 0, ce = 0;
+    if (lifted) { cs = g.cptr[v]; ce = g.cptr[v + 1]; }
+    if (lifted && gl == 0) acc.preds += (uint32_t)(ce - cs);
+    const int tag = r + 1;
+    for (int k = cs + gl; k < ce; k += G) {
+        const int u = g.cpred[k];
+        bool first = false;
+        if (g.stamp[u] < tag) first = atomicMax(&g.stamp[u], tag) < tag;
+        stage_push(g, st, first, u);
+    }
+}
+
+// Hand a vertex to the wavefront / workgroup kernel of this round (sparse modes).
+__device__ __forceinline__ void defer(const DevGame &g, int r, int v, int deg) {
+    if (deg > g.huge_deg) {
+        const int pos = atomicAdd(&g.ring[6 + r % 3], 1);
+        g.dhuge[pos] = v;
+    } else {
+        const int pos = atomicAdd(&g.ring[3 + r % 3], 1);
+        g.dbig[pos] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// initial frontier
+// ---------------------------------------------------------------------------------------
+
+// lift(0)(v) > 0 exactly for an owner-0 vertex with all out-weights negative or an
+// owner-1 vertex with a negative out-weight (src/dev_EG_alg.cu:176-205,
+// src/cpu_solver.c:217-249).  Values start at 0; the frontier becomes round 1's queue.
+__global__ __launch_bounds__(256) void k_init(DevGame g) {
+    __shared__ int sbuf[kLdsCap];
+    __shared__ int scnt;
+    if (threadIdx.x == 0) scnt = 0;
+    __syncthreads();
+    const Stage st{sbuf, &scnt, g.list[1], g.lcnt + 1 * kListCounters, (int)(blockIdx.x % kSub)};
+    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x; v0 < g.n; v0 += (int64_t)gridDim.x * blockDim.x) {
+        const int v = (int)(v0 + threadIdx.x);
+        bool act = false;
+        if (v < g.n) {
+            const int s = g.ptr[v], e = g.ptr[v + 1];
+            bool all_neg = true, any_neg = false;
+            for (int k = s; k < e; k++) {
+                const bool neg = g.edges[k].y < 0;
+                all_neg &= neg;
+                any_neg |= neg;
+            }
+            act = (v < g.n0) ? all_neg : any_neg;
+            g.val[v] = 0;
+            g.stamp[v] = act ? 1 : 0;
+        }
+        stage_push(g, st, act, v);
+        stage_flush(g, st, kLdsCap / 2);
+    }
+    stage_flush(g, st, 1);
+}
+
+// ---------------------------------------------------------------------------------------
+// round kernels
+// ---------------------------------------------------------------------------------------
+
+template <int G>
+__device__ __forceinline__ void body_dense(const DevGame &g, const Stage &st, LiftAcc &acc) {
+    constexpr int ITEMS = 256 / G;
+    const int gl = threadIdx.x % G;
+    int it = 0;
+    for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < g.n; t0 += (int64_t)gridDim.x * ITEMS, it++) {
+        const int v = (int)(t0 + threadIdx.x / G);
+        bool valid = v < g.n;
+        int s = 0, e = 0, old = 0;
+        if (valid) {
+            s = g.ptr[v];
+            e = g.ptr[v + 1];
+            if (e - s > g.big_deg) { valid = false; s = e = 0; }   // static wave/block lists
+            else old = g.val[v];
+        }
+        const int t = lift_value<G>(g, s, e, gl, v >= g.n0);
+        const bool lifted = valid && old < t;
+        if (valid && gl == 0) {
+            acc.processed++;
+            acc.edges += (uint32_t)(e - s);
+            if (lifted) { g.val[v] = t; acc.lifted++; }
+        }
+        stage_push(g, st, lifted && gl == 0, v);
+        if ((it & 7) == 7) stage_flush(g, st, kLdsCap / 2);
+    }
+}
+
+template <int G>
+__device__ __forceinline__ void body_queue(const DevGame &g, const Stage &st, const int *in_list,
+                                           const int *pre, int r, LiftAcc &acc) {
+    constexpr int ITEMS = 256 / G;
+    const int gl = threadIdx.x % G;
+    const int total = pre[kSub + 1];
+    int it = 0;
+    for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < total; t0 += (int64_t)gridDim.x * ITEMS, it++) {
+        const int64_t item = t0 + threadIdx.x / G;
+        bool valid = item < total;
+        int v = 0, s = 0, e = 0, old = 0;
+        if (valid) {
+            v = in_list[layout_slot(g, pre, (int)item)];
+            s = g.ptr[v];
+            e = g.ptr[v + 1];
+            if (e - s > g.big_deg) {
+                if (gl == 0) defer(g, r, v, e - s);
+                valid = false;
+                s = e = 0;
+            } else {
+                old = g.val[v];
+            }
+        }
+        const int t = lift_value<G>(g, s, e, gl, v >= g.n0);
+        const bool lifted = valid && old < t;
+        if (valid && gl == 0) {
+            acc.processed++;
+            acc.edges += (uint32_t)(e - s);
+            if (lifted) { g.val[v] = t; acc.lifted++; }
+        }
+        push_preds<G>(g, st, v, lifted, gl, r, acc);
+        if ((it & 3) == 3) stage_flush(g, st, kLdsCap / 2);
+    }
+}
+
+// Items are vertices that rose in the previous (dense) round.  Each predecessor p of an
+// item is claimed once (stamp[p] -> r) and lifted by the claiming lane.
+template <int G>
+__device__ __forceinline__ void body_lifted(const DevGame &g, const Stage &st, const int *in_list,
+                                            const int *pre, int r, LiftAcc &acc) {
+    constexpr int ITEMS = 256 / G;
+    const int gl = threadIdx.x % G;
+    const int total = pre[kSub + 1];
+    int it = 0;
+    for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < total; t0 += (int64_t)gridDim.x * ITEMS, it++) {
+        const int64_t item = t0 + threadIdx.x / G;
+        int cs = 0, ce = 0;
+        if (item < total) {
+            const int u = in_list[layout_slot(g, pre, (int)item)];
+            cs = g.cptr[u];
+            ce = g.cptr[u + 1];
+        }
+        for (int k = cs + gl; k < ce; k += G) {
+            const int p = g.cpred[k];
+            bool mine = false;
+            if (g.stamp[p] < r) mine = atomicMax(&g.stamp[p], r) < r;
+            if (mine) {
+                const int s = g.ptr[p], e = g.ptr[p + 1];
+                if (e - s > g.big_deg) {
+                    defer(g, r, p, e - s);
+                } else {
+                    const int old = g.val[p];
+                    const int t = lift_value<1>(g, s, e, 0, p >= g.n0);
+                    acc.processed++;
+                    acc.edges += (uint32_t)(e - s);
+                    if (old < t) {
+                        g.val[p] = t;
+                        acc.lifted++;
+                        const int qs = g.cptr[p], qe = g.cptr[p + 1];
+                        acc.preds += (uint32_t)(qe - qs);
+                        for (int q = qs; q < qe; q++) {
+                            const int u2 = g.cpred[q];
+                            bool first = false;
+                            if (g.stamp[u2] < r + 1) first = atomicMax(&g.stamp[u2], r + 1) < r + 1;
+                            stage_push(g, st, first, u2);
+                        }
+                    }
+                }
+            }
+        }
+        if ((it & 3) == 3) stage_flush(g, st, kLdsCap / 2);
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void k_round(DevGame g, int r) {
+    __shared__ int pre[kSub + 2];
+    __shared__ int sbuf[kLdsCap];
+    __shared__ int scnt, smode;
+    const int in_slot = r % 3;
+    if (threadIdx.x == 0) scnt = 0;
+    load_layout(g, in_slot, pre);
+    __syncthreads();
+    if (threadIdx.x == 0) smode = decide_mode(g, r, pre[kSub + 1]);
+    __syncthreads();
+    const int mode = smode;
+    if (blockIdx.x == 0) {
+        // prepare round r+1: its output counters and its deferral counters
+        int *z = g.lcnt + ((r + 2) % 3) * kListCounters;
+        for (int i = threadIdx.x; i < kListCounters; i += blockDim.x) z[i] = 0;
+        if (threadIdx.x == 0) {
+            g.mode[in_slot] = mode;
+            g.ring[3 + (r + 1) % 3] = 0;
+            g.ring[6 + (r + 1) % 3] = 0;
+            if (mode != MODE_DONE) *g.rounds_done = r;
+        }
+    }
+    if (mode == MODE_DONE) return;
+    const Stage st{sbuf, &scnt, g.list[(r + 1) & 1], g.lcnt + ((r + 1) % 3) * kListCounters,
+                   (int)(blockIdx.x % kSub)};
+    LiftAcc acc;
+    if (mode == MODE_DENSE) body_dense<G>(g, st, acc);
+    else if (mode == MODE_QUEUE) body_queue<G>(g, st, g.list[r & 1], pre, r, acc);
+    else body_lifted<G>(g, st, g.list[r & 1], pre, r, acc);
+    stage_flush(g, st, 1);
+    flush_stats(g, acc);
+}
+
+// One wavefront per vertex: the static big list in dense rounds, the deferred list in
+// sparse rounds.
+__global__ __launch_bounds__(256) void k_lift_wave(DevGame g, int r) {
+    __shared__ int sbuf[kLdsCap];
+    __shared__ int scnt;
+    const int mode = g.mode[r % 3];
+    if (mode == MODE_DONE) return;
+    const bool dense = mode == MODE_DENSE;
+    const int *items = dense ? g.sbig : g.dbig;
+    const int cnt = dense ? g.ring[9] : g.ring[3 + r % 3];
+    if (cnt == 0) return;
+    if (threadIdx.x == 0) scnt = 0;
+    __syncthreads();
+    const Stage st{sbuf, &scnt, g.list[(r + 1) & 1], g.lcnt + ((r + 1) % 3) * kListCounters,
+                   (int)(blockIdx.x % kSub)};
+    LiftAcc acc;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t i = wave; i < cnt; i += waves) {
+        const int v = items[i];
+        const int s = g.ptr[v], e = g.ptr[v + 1];
+        const int old = g.val[v];
+        const int t = lift_value<64>(g, s, e, lane, v >= g.n0);
+        const bool lifted = old < t;
+        if (lane == 0) {
+            acc.processed++;
+            acc.edges += (uint32_t)(e - s);
+            if (lifted) { g.val[v] = t; acc.lifted++; }
+        }
+        if (dense) stage_push(g, st, lifted && lane == 0, v);
+        else push_preds<64>(g, st, v, lifted, lane, r, acc);
+    }
+    stage_flush(g, st, 1);
+    flush_stats(g, acc);
+}
+
+// One workgroup (256 lanes) per vertex: the static huge list in dense rounds, the
+// deferred list in sparse rounds.
+__global__ __launch_bounds__(256) void k_lift_block(DevGame g, int r) {
+    __shared__ int sbuf[kLdsCap];
+    __shared__ int scnt;
+    __shared__ int wmin[4];
+    const int mode = g.mode[r % 3];
+    if (mode == MODE_DONE) return;
+    const bool dense = mode == MODE_DENSE;
+    const int *items = dense ? g.shuge : g.dhuge;
+    const int cnt = dense ? g.ring[10] : g.ring[6 + r % 3];
+    if (cnt == 0) return;
+    if (threadIdx.x == 0) scnt = 0;
+    __syncthreads();
+    const Stage st{sbuf, &scnt, g.list[(r + 1) & 1], g.lcnt + ((r + 1) % 3) * kListCounters,
+                   (int)(blockIdx.x % kSub)};
+    LiftAcc acc;
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    for (int64_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+        const int v = items[i];
+        const int s = g.ptr[v], e = g.ptr[v + 1];
+        const int old = g.val[v];
+        const bool owner1 = v >= g.n0;
+        int key = EGS_TOP;
+        for (int k = s + (int)threadIdx.x; k < e; k += 256) {
+            const int2 ew = g.edges[k];
+            key = min(key, egs_key(egs_ominus(g.val[ew.x], ew.y, g.mg), owner1));
+        }
+        key = egs_group_min<64>(key);
+        if (lane == 0) wmin[wid] = key;
+        __syncthreads();
+        key = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
+        __syncthreads();
+        const int t = egs_unkey(key, owner1);
+        const bool lifted = old < t;
+        if (threadIdx.x == 0) {
+            acc.processed++;
+            acc.edges += (uint32_t)(e - s);
+            if (lifted) { g.val[v] = t; acc.lifted++; }
+        }
+        if (dense) stage_push(g, st, lifted && threadIdx.x == 0, v);
+        else push_preds<256>(g, st, v, lifted, (int)threadIdx.x, r, acc);
+        stage_flush(g, st, kLdsCap / 2);
+    }
+    stage_flush(g, st, 1);
+    flush_stats(g, acc);
+}
+
+// ---------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------
+
+hipError_t launch_init(const DevGame &g, hipStream_t st) {
+    hipError_t err = hipMemsetAsync(g.lcnt, 0, sizeof(int) * 3 * kListCounters, st);
+    if (err == hipSuccess) err = hipMemsetAsync(g.mode, 0, sizeof(int) * 3, st);   // MODE_QUEUE
+    if (err == hipSuccess) err = hipMemsetAsync(g.ring, 0, sizeof(int) * 9, st);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_init, dim3(grid_for(g.n, 256, 2048)), dim3(256), 0, st, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int r, hipStream_t st) {
+    switch (sh.group) {
+    case 1: hipLaunchKernelGGL(k_round<1>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
+    case 2: hipLaunchKernelGGL(k_round<2>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
+    case 4: hipLaunchKernelGGL(k_round<4>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
+    case 8: hipLaunchKernelGGL(k_round<8>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
+    case 16: hipLaunchKernelGGL(k_round<16>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
+    case 32: hipLaunchKernelGGL(k_round<32>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
+    default: hipLaunchKernelGGL(k_round<64>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
+    }
+    if (sh.use_wave) hipLaunchKernelGGL(k_lift_wave, dim3(sh.wave_grid), dim3(256), 0, st, g, r);
+    if (sh.use_block) hipLaunchKernelGGL(k_lift_block, dim3(sh.block_grid), dim3(256), 0, st, g, r);
+    return hipGetLastError();
+}
+
+}  // namespace egs

@@ -15,6 +15,7 @@
 #ifndef EGS_H
 #define EGS_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -42,7 +43,9 @@ typedef struct egs_options {
     int32_t huge_deg;       /* 0 = auto */
     int32_t grid;           /* workgroups of the group kernel; 0 = auto */
     int32_t batch_rounds;   /* rounds enqueued between host convergence checks; 0 = auto */
-    int32_t reserved0;
+    int32_t dense_pct;      /* a round sweeps every vertex when its estimated work is at
+                               least this percent of n (else it walks the worklist);
+                               0 = auto (12), > 100 = never sweep */
     int64_t max_phases;     /* phase cap; < 0 = the reference bound n*m*MG+1
                                (src/egsolver.c:64-90) */
     const volatile uint32_t *stop_flag;  /* polled between batches; non-zero stops the
@@ -79,11 +82,21 @@ typedef struct egs_device_info {
     int32_t ecc;
     int32_t driver_version;
     int32_t runtime_version;
+    int32_t cap_major;
+    int32_t cap_minor;
 } egs_device_info;
 
 int egs_device_query(int device, egs_device_info *out);
 int egs_device_reset(int device);
 const char *egs_last_error(void);
+
+/* Pinned (page-locked) host memory, as cudaMallocHost/cudaFreeHost in the reference
+ * (src/dev_solver.cu:156,244). */
+int egs_host_alloc(void **p, size_t bytes);
+int egs_host_free(void *p);
+/* Pops the HIP runtime's sticky error: 0 when none, else the code with its text in buf
+ * (replaces cudaGetLastError in checkCUDAError, src/utils.cu:38-45). */
+int egs_runtime_error(char *buf, int len);
 
 /* ---- one game per solver --------------------------------------------------------- */
 
@@ -121,6 +134,13 @@ int egs_solve(int device, int32_t n, int32_t n0, int64_t m, int32_t mg, const in
 /* Fixpoint certificate on the device: number of vertices with lift(val)(v) != val(v).
  * Zero, for values reached by monotone lifting from 0, certifies the least fixpoint. */
 int egs_solver_count_violations(egs_solver *s, int64_t *violations);
+
+/* ---- calibration ------------------------------------------------------------------ */
+
+/* Time `reps` launches of a plain 16-byte-vector device copy of `bytes` bytes and report
+ * (read + write bytes) / time in GB/s.  Used to state the achievable HBM rate beside the
+ * vendor peak and to scale profiler byte counters. */
+int egs_copy_bandwidth(int device, int64_t bytes, int reps, double *gbps);
 
 #ifdef __cplusplus
 }

@@ -33,3 +33,14 @@ def load_golden_sol(name="clique_500_cp4.sol"):
 def to_oracle_game(g):
     from oracle.oracle import game_from_arrays
     return game_from_arrays(g.n, g.n0, g.ptr, g.succ, g.w, g.ext_of_int)
+
+
+def lfp(oracle, og, threads=4):
+    """Least fixpoint from the oracle: EG0 (Jacobi), cross-checked against the EG
+    worklist whenever its all-active-start defect cannot apply."""
+    import numpy as np
+    v0, _ = oracle.eg0(og, threads=threads)
+    if not oracle.initial_active(og).all():
+        v1, _ = oracle.eg(og)
+        assert np.array_equal(v0, v1), "oracle EG and EG0 disagree"
+    return v0

@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, load_golden_sol, to_oracle_game
+from conftest import GOLDEN, lfp, load_golden_sol, to_oracle_game
 
 pytestmark = pytest.mark.gpu
 
@@ -37,7 +37,7 @@ def test_eg_fig2_family(oracle):
         got = dict(zip(g.ext_of_int.tolist(), vals.tolist()))
         for ext, v in got.items():
             assert v == hand[ext % 5], (name, ext, v)
-        ref, _ = oracle.eg(to_oracle_game(g))
+        ref = lfp(oracle, to_oracle_game(g))
         assert np.array_equal(vals, ref)
 
 
@@ -59,11 +59,12 @@ def test_config1_vs_oracle(oracle):
     import egsolver_amd as E
     _say("config1 n=2000 deg=4 W=10 seed=1")
     g = E.config_game(1)
-    ref, _ = oracle.eg(to_oracle_game(g))
+    ref = lfp(oracle, to_oracle_game(g))
     for group in (1, 2, 4, 8, 16, 32, 64):
-        vals, st = _solve(g, group=group)
-        assert st["converged"]
-        assert np.array_equal(vals, ref), f"group={group}: {np.count_nonzero(vals != ref)} differ"
+        for dense_pct in (0, 1, 101):
+            vals, st = _solve(g, group=group, dense_pct=dense_pct)
+            assert st["converged"]
+            assert np.array_equal(vals, ref), f"group={group} dense_pct={dense_pct}: {np.count_nonzero(vals != ref)} differ"
 
 
 @pytest.mark.parametrize("seed", range(6))
@@ -78,11 +79,12 @@ def test_random_small_all_paths(oracle, seed):
                     alpha=1.6, wmax=int(rng.integers(1, 50)))
     g = E.generate(p)
     _say(f"random seed={seed} n={g.n} m={g.m} maxdeg={g.max_outdeg}")
-    ref, _ = oracle.eg(to_oracle_game(g))
+    ref = lfp(oracle, to_oracle_game(g))
     for group, big, huge in [(1, 4, 16), (4, 8, 64), (8, 16, 32), (2, 0, 0)]:
-        vals, st = _solve(g, group=group, big_deg=big, huge_deg=huge)
-        assert st["converged"]
-        assert np.array_equal(vals, ref), (group, big, huge, np.count_nonzero(vals != ref))
+        for dense_pct in (0, 1, 30, 101):
+            vals, st = _solve(g, group=group, big_deg=big, huge_deg=huge, dense_pct=dense_pct)
+            assert st["converged"]
+            assert np.array_equal(vals, ref), (group, big, huge, dense_pct, np.count_nonzero(vals != ref))
 
 
 def test_transpose_matches_oracle(oracle):
@@ -106,10 +108,11 @@ def test_scaled_configs_vs_oracle(oracle, which, scale):
     import egsolver_amd as E
     g = E.config_game(which, scale=scale)
     _say(f"config{which} scaled n={g.n} m={g.m}")
-    ref, _ = oracle.eg(to_oracle_game(g))
-    vals, st = _solve(g)
-    assert st["converged"]
-    assert np.array_equal(vals, ref), np.count_nonzero(vals != ref)
+    ref = lfp(oracle, to_oracle_game(g))
+    for dense_pct in (0, 101):
+        vals, st = _solve(g, dense_pct=dense_pct)
+        assert st["converged"]
+        assert np.array_equal(vals, ref), (dense_pct, np.count_nonzero(vals != ref))
     with E.Solver(g.n, g.m) as s:
         s.load(g)
         s.prepare()
@@ -123,7 +126,7 @@ def test_partial_run_is_under_approximation(oracle):
     import egsolver_amd as E
     g = E.config_game(2, scale=1 / 1024)
     _say(f"partial n={g.n}")
-    ref, _ = oracle.eg(to_oracle_game(g))
+    ref = lfp(oracle, to_oracle_game(g))
     with E.Solver(g.n, g.m) as s:
         s.load(g)
         s.prepare()
