@@ -8,3 +8,5 @@ tests and the benchmark.
 """
 from .games import Game, GenParams, config_game, generate, family_params, read_mpg, write_mpg, TOP  # noqa: F401
 from .solver import Solver, solve, options, device_info  # noqa: F401
+from .batch import Batch, batch_options  # noqa: F401,E402
+from .serve import Server, Request, serve_options, run_served, run_plain, latency_stats  # noqa: F401,E402

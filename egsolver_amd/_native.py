@@ -38,6 +38,8 @@ class EgsOptions(ctypes.Structure):
         ("grid", ctypes.c_int32),
         ("batch_rounds", ctypes.c_int32),
         ("dense_pct", ctypes.c_int32),
+        ("certify_every", ctypes.c_int32),
+        ("cert_max_passes", ctypes.c_int32),
         ("max_phases", ctypes.c_int64),
         ("stop_flag", ctypes.c_void_p),
     ]
@@ -54,6 +56,10 @@ class EgsStats(ctypes.Structure):
         ("initial_active", ctypes.c_int32),
         ("ms_prepare", ctypes.c_double),
         ("ms_iterate", ctypes.c_double),
+        ("cert_attempts", ctypes.c_int64),
+        ("certificates", ctypes.c_int64),
+        ("cert_passes", ctypes.c_int64),
+        ("top_marked", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -62,6 +62,8 @@ constexpr int kPinConv = 448;     // kListCounters ints
 constexpr int kPinRounds = 576;
 constexpr int kPinStats = 640;    // 4 x u64
 constexpr int kPinViol = 704;     // 1 x u64
+constexpr int kPinCert = 1024;    // certification readbacks (2 x 64 + 16 ints)
+constexpr int kPinForce = 1280;   // round number forced to a full sweep
 
 int pow2ceil(int x) {
     int p = 1;
@@ -83,6 +85,11 @@ struct egs_solver {
     int *d_sbig = nullptr, *d_shuge = nullptr, *d_lcnt = nullptr, *d_mode = nullptr;
     int *d_ring = nullptr, *d_rounds = nullptr, *d_misc = nullptr, *d_tile = nullptr, *d_fill = nullptr;
     int sub_cap = 0;
+    int *d_sigma = nullptr, *d_hist = nullptr, *d_bfcnt = nullptr;
+    long long *d_h = nullptr;
+    int wmax = 0;
+    int cert_every = 32, cert_gap = 32, cert_max_passes = 1024;
+    int64_t next_cert = 0;
     int2 *d_edges = nullptr;
     unsigned long long *d_stats = nullptr, *d_statsum = nullptr;
     int *h_probe = nullptr;   // pinned: per-batch counter probes + misc readbacks
@@ -205,7 +212,8 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
            o_lcnt = take(4 * 3 * egs::kListCounters), o_mode = take(4 * 4),
            o_ring = take(4 * 16), o_rounds = take(4 * 16), o_misc = take(8 * 16),
            o_tile = take(4 * (size_t)tiles), o_fill = take(4 * n1),
-           o_stats = take(8 * 4 * (size_t)egs::kStatSlots), o_statsum = take(8 * 8);
+           o_stats = take(8 * 4 * (size_t)egs::kStatSlots), o_statsum = take(8 * 8),
+           o_sigma = take(4 * n1), o_h = take(8 * n1), o_hist = take(4 * 256), o_bfcnt = take(4 * 64);
     s->block_bytes = off;
     if (hipMalloc(&s->block, off) != hipSuccess) {
         egs_solver_destroy(s);
@@ -233,6 +241,10 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
     s->d_fill = (int *)(b + o_fill);
     s->d_stats = (unsigned long long *)(b + o_stats);
     s->d_statsum = (unsigned long long *)(b + o_statsum);
+    s->d_sigma = (int *)(b + o_sigma);
+    s->d_h = (long long *)(b + o_h);
+    s->d_hist = (int *)(b + o_hist);
+    s->d_bfcnt = (int *)(b + o_bfcnt);
     if (hipHostMalloc((void **)&s->h_probe, 8192, hipHostMallocDefault) != hipSuccess) {
         egs_solver_destroy(s);
         return fail(EGS_E_NOMEM, "hipHostMalloc failed");
@@ -270,6 +282,9 @@ int egs_solver_load(egs_solver *s, int mem, int32_t n, int32_t n0, int64_t m, in
         for (int64_t e = 0; e < m; e++)
             if (succ[e] < 0 || succ[e] >= n) return fail(EGS_E_ARG, "edge %lld: successor %d out of range", (long long)e, succ[e]);
         s->max_outdeg = mx;
+        int wm = 0;
+        for (int64_t e = 0; e < m; e++) wm = std::max(wm, w[e] < 0 ? -w[e] : w[e]);
+        s->wmax = wm;
         int2 *packed = (int2 *)malloc(sizeof(int2) * (size_t)m);
         if (!packed) return fail(EGS_E_NOMEM, "host staging allocation failed");
         for (int64_t e = 0; e < m; e++) packed[e] = make_int2(succ[e], w[e]);
@@ -286,6 +301,13 @@ int egs_solver_load(egs_solver *s, int mem, int32_t n, int32_t n0, int64_t m, in
         HIP_TRY(hipMemcpyAsync(s->h_probe + kPinDeg, s->d_misc, sizeof(int), hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
         s->max_outdeg = s->h_probe[kPinDeg];
+        egs::DevGame tmp{};
+        tmp.edges = s->d_edges;
+        tmp.m = m;
+        HIP_TRY(egs::launch_abs_wmax(tmp, s->d_misc, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->h_probe + kPinDeg, s->d_misc, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        s->wmax = s->h_probe[kPinDeg];
     } else {
         return fail(EGS_E_ARG, "mem must be EGS_MEM_HOST or EGS_MEM_DEVICE");
     }
@@ -342,6 +364,8 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     s->batch_rounds = o->batch_rounds > 0 ? o->batch_rounds : 32;
     s->phase_cap = egs_max_loop_c(g.m, g.n, g.mg, o->max_phases);
     s->stop_flag = o->stop_flag;
+    s->cert_every = o->certify_every == 0 ? 32 : (o->certify_every < 0 ? 0 : o->certify_every);
+    s->cert_max_passes = o->cert_max_passes > 0 ? o->cert_max_passes : 1024;
     return EGS_OK;
 }
 
@@ -356,6 +380,7 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     HIP_TRY(egs::launch_transpose(s->g, s->d_tile, s->d_fill, s->stream));
     HIP_TRY(egs::launch_classify(s->g, s->d_ring + 9, s->stream));
     HIP_TRY(egs::launch_init(s->g, s->stream));
+    HIP_TRY(hipMemsetAsync(s->d_ring + 11, 0, sizeof(int), s->stream));
     HIP_TRY(hipEventRecord(s->ev_b, s->stream));
     HIP_TRY(hipMemcpyAsync(s->h_probe + kPinInit, s->d_lcnt + 1 * egs::kListCounters,
                            sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
@@ -366,6 +391,8 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     s->st.ms_prepare = ms;
     s->st.initial_active = (int32_t)list_total(s, s->h_probe + kPinInit);
     s->next_round = 1;
+    s->cert_gap = s->cert_every;
+    s->next_cert = s->cert_every;
     s->prepared = true;
     return EGS_OK;
 }
@@ -399,7 +426,35 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
     bool done = false;
     int batch = 4;        // grow towards batch_rounds so short solves do not overshoot much
     while (!done && launched < limit) {
-        const int nb = (int)std::min<int64_t>(batch, limit - launched);
+        int nb = (int)std::min<int64_t>(batch, limit - launched);
+        const bool accel = s->cert_every > 0;
+        if (accel && s->next_round - 1 >= s->next_cert) {
+            // pause: converged already?  else try to prove a climbing set is TOP
+            HIP_TRY(hipMemcpyAsync(s->h_probe + kPinConv, s->d_lcnt + (s->next_round % 3) * egs::kListCounters,
+                                   sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            if (list_total(s, s->h_probe + kPinConv) == 0) break;
+            egs::CertBuffers cb{s->d_hist, s->d_sigma, s->d_h, s->d_bfcnt, s->h_probe + kPinCert};
+            egs::CertResult cr;
+            const int got = egs::certify_top(s->g, cb, s->wmax, s->cert_max_passes, s->stream, cr);
+            if (got < 0) return fail(EGS_E_HIP, "certification: %s", hipGetErrorString(hipGetLastError()));
+            s->st.cert_attempts++;
+            s->st.cert_passes += cr.passes;
+            if (got > 0) {
+                s->st.certificates++;
+                s->st.top_marked += cr.marked;
+                s->h_probe[kPinForce] = (int)s->next_round;           // next round sweeps everything
+                HIP_TRY(hipMemcpyAsync(s->d_ring + 11, s->h_probe + kPinForce, sizeof(int), hipMemcpyHostToDevice,
+                                       s->stream));
+                HIP_TRY(hipStreamSynchronize(s->stream));
+                s->cert_gap = s->cert_every;
+            } else {
+                s->cert_gap = std::min(s->cert_gap * 2, 1 << 16);
+            }
+            s->next_cert = s->next_round - 1 + s->cert_gap;
+            pending = -1;
+        }
+        if (accel) nb = (int)std::max<int64_t>(1, std::min<int64_t>(nb, s->next_cert - (s->next_round - 1)));
         for (int k = 0; k < nb; k++) {
             HIP_TRY(egs::launch_round(s->g, s->sh, (int)s->next_round, s->stream));
             s->next_round++;

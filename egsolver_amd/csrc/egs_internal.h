@@ -33,7 +33,8 @@ struct DevGame {
     int *lcnt;            // 3 slots x kListCounters
     int *mode;            // 3 slots: mode of round r at r % 3
     int *dbig, *dhuge;    // per-round deferral lists (sparse modes)
-    int *ring;            // [3..5] dbig counters, [6..8] dhuge counters, [9] nsbig, [10] nshuge
+    int *ring;            // [3..5] dbig counters, [6..8] dhuge counters, [9] nsbig, [10] nshuge,
+                          // [11] a round that must sweep all vertices (after a TOP proof)
     const int *sbig;      // static: vertices with big_deg < deg <= huge_deg
     const int *shuge;     // static: vertices with deg > huge_deg
     int big_deg, huge_deg;
@@ -59,5 +60,19 @@ hipError_t launch_sum_stats(const DevGame &g, int nslots, unsigned long long *ou
 hipError_t launch_degree_stats(const int *ptr, int n, int *out, hipStream_t st);
 hipError_t launch_violations(const DevGame &g, unsigned long long *out, hipStream_t st);
 int grid_for(int64_t work, int threads, int cap);
+
+// Certified TOP detection (egs_accel.hip).
+struct CertBuffers {
+    int *hist;        // device: 2 x 64 histogram bins + 8 flags
+    int *sigma;       // device: n
+    long long *h;     // device: n
+    int *bfcnt;       // device: 8 pass counters
+    int *pinned;      // host pinned: 2 x 64 + 16 ints
+};
+struct CertResult {
+    int threshold, passes, marked, closure_failed;
+};
+hipError_t launch_abs_wmax(const DevGame &g, int *out, hipStream_t st);
+int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int max_passes, hipStream_t st, CertResult &res);
 
 }  // namespace egs

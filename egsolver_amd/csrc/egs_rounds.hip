@@ -133,6 +133,7 @@ __device__ __forceinline__ int layout_slot(const DevGame &g, const int *pre, int
 
 // Round mode from round r-1's mode and output size (identical in every workgroup).
 __device__ __forceinline__ int decide_mode(const DevGame &g, int r, int total) {
+    if (g.ring[11] == r) return MODE_DENSE;     // values were raised outside the rounds
     const int prev = g.mode[(r + 2) % 3];
     if (prev == MODE_DONE || total == 0) return MODE_DONE;
     if (prev == MODE_DENSE) {

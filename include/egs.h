@@ -46,6 +46,10 @@ typedef struct egs_options {
     int32_t dense_pct;      /* a round sweeps every vertex when its estimated work is at
                                least this percent of n (else it walks the worklist);
                                0 = auto (12), > 100 = never sweep */
+    int32_t certify_every;  /* rounds between attempts to prove climbing vertices TOP
+                               (exact; attempts back off while they fail);
+                               0 = auto (32), < 0 = never */
+    int32_t cert_max_passes;/* longest-path passes per attempt; 0 = auto (1024) */
     int64_t max_phases;     /* phase cap; < 0 = the reference bound n*m*MG+1
                                (src/egsolver.c:64-90) */
     const volatile uint32_t *stop_flag;  /* polled between batches; non-zero stops the
@@ -62,7 +66,11 @@ typedef struct egs_stats {
     int32_t converged;      /* 1 when the frontier emptied (values are the fixpoint) */
     int32_t initial_active; /* size of the initial frontier */
     double ms_prepare;      /* pack + transpose + initial frontier (device time) */
-    double ms_iterate;      /* fixpoint rounds (device time) */
+    double ms_iterate;      /* fixpoint rounds and proofs (device time) */
+    int64_t cert_attempts;  /* TOP proofs attempted */
+    int64_t certificates;   /* TOP proofs that succeeded */
+    int64_t cert_passes;    /* longest-path passes spent on proofs */
+    int64_t top_marked;     /* vertices set to TOP by proofs */
 } egs_stats;
 
 /* ---- device ---------------------------------------------------------------------- */
@@ -134,6 +142,95 @@ int egs_solve(int device, int32_t n, int32_t n0, int64_t m, int32_t mg, const in
 /* Fixpoint certificate on the device: number of vertices with lift(val)(v) != val(v).
  * Zero, for values reached by monotone lifting from 0, certifies the least fixpoint. */
 int egs_solver_count_violations(egs_solver *s, int64_t *violations);
+
+/* ---- many games in one launch ---------------------------------------------------- */
+
+/* No reference counterpart (the reference solves one game per process).  Games are
+ * staged with egs_batch_add (internal numbering, as above), copied with
+ * egs_batch_upload, and solved together by one persistent kernel launch in which each
+ * workgroup takes whole games from a device counter, largest first. */
+typedef struct egs_batch egs_batch;
+
+typedef struct egs_batch_options {
+    int32_t threads;            /* workgroup size: 256, 512 or 1024; 0 = 512 */
+    int32_t blocks_per_cu;      /* persistent workgroups per compute unit; 0 = 2 */
+    int32_t lds_ints;           /* LDS values per workgroup: games up to this many vertices
+                                   keep their values in LDS; 0 = 16384 */
+    int32_t lanes_per_edge_x16; /* lanes per vertex = smallest power of two >= average
+                                   degree * this / 16; 0 = 8 */
+    int32_t group_min, group_max; /* clamp of the lanes per vertex; 0 = 1 / 64 */
+    int64_t max_sweeps;         /* per-game sweep cap; <= 0 = none */
+    int32_t certify_every;      /* sweeps between TOP proofs per game; 0 = 16, < 0 = never */
+    int32_t cert_max_passes;    /* longest-path passes per proof; 0 = 1024 */
+} egs_batch_options;
+
+typedef struct egs_batch_stats {
+    int64_t games;
+    int64_t sweeps;             /* summed over games */
+    int64_t edges_relaxed;
+    int64_t lifted;
+    int64_t converged;          /* games that reached their fixpoint */
+    double ms;                  /* device time of the launch */
+    int32_t packed_edges;       /* 1 when edges travelled as 32-bit words */
+    int32_t grid, threads, lds_ints;
+    int64_t certificates;       /* TOP proofs that succeeded, summed over games */
+    int64_t cert_passes;        /* longest-path passes spent on proofs */
+} egs_batch_stats;
+
+int egs_batch_create(int device, egs_batch **out);
+int egs_batch_destroy(egs_batch *b);
+int egs_batch_clear(egs_batch *b);
+/* Returns the game's index in the batch (>= 0) or an error code. */
+int egs_batch_add(egs_batch *b, int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *ptr,
+                  const int32_t *succ, const int32_t *w);
+int egs_batch_size(egs_batch *b);
+int egs_batch_upload(egs_batch *b);
+int egs_batch_solve(egs_batch *b, const egs_batch_options *opt, egs_batch_stats *st);
+int egs_batch_read_values(egs_batch *b, int game, int32_t *out);
+int egs_batch_read_all(egs_batch *b, int32_t *out);
+int egs_batch_game_stats(egs_batch *b, int game, int64_t *sweeps, int64_t *edges, int32_t *converged);
+const char *egs_batch_last_error(void);
+
+/* ---- serving ------------------------------------------------------------------------ */
+
+/* No reference counterpart.  A server owns a device arena.  submit() admits a game when
+ * its footprint fits (else EGS_E_CAPACITY) and uploads it on a copy stream that
+ * overlaps any running slice; launch() runs one time slice of the many-games kernel over
+ * every resident unfinished game (graph, values and proof state stay resident between
+ * slices); wait() ends the slice and lists the games that reached their fixpoint;
+ * fetch() returns a finished game's values and frees its memory.  One thread drives a
+ * server. */
+typedef struct egs_server egs_server;
+
+typedef struct egs_serve_options {
+    int32_t threads;            /* 256, 512 or 1024; 0 = 512 */
+    int32_t blocks_per_cu;      /* 0 = 2 */
+    int32_t lds_ints;           /* 0 = 16384 */
+    int32_t lanes_per_edge_x16; /* 0 = 8 */
+    int32_t group_min, group_max;
+    int32_t slice_sweeps;       /* sweeps per game per slice; 0 = 64 */
+    int32_t certify_every;      /* 0 = 16, < 0 = never */
+    int32_t cert_max_passes;    /* 0 = 1024 */
+    int32_t reserved;
+} egs_serve_options;
+
+int egs_server_create(int device, int64_t arena_bytes, int32_t max_games, egs_server **out);
+int egs_server_destroy(egs_server *s);
+/* Device bytes a game of n vertices, m edges and max |w| occupies in the arena. */
+int64_t egs_server_footprint(int32_t n, int64_t m, int32_t wmax);
+int64_t egs_server_free_bytes(egs_server *s);
+int egs_server_free_slots(egs_server *s);
+/* Returns a ticket >= 0, or EGS_E_CAPACITY when the game does not fit right now. */
+int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *ptr,
+                      const int32_t *succ, const int32_t *w);
+/* Returns the number of games in the slice started (0: nothing resident). */
+int egs_server_launch(egs_server *s, const egs_serve_options *opt);
+/* Returns how many games finished in the slice; their tickets go to done[0..cap). */
+int egs_server_wait(egs_server *s, int32_t *done, int32_t cap);
+double egs_server_last_slice_ms(egs_server *s);
+int egs_server_status(egs_server *s, int ticket);
+int egs_server_fetch(egs_server *s, int ticket, int32_t *values, int64_t *sweeps, int64_t *edges);
+const char *egs_server_last_error(void);
 
 /* ---- calibration ------------------------------------------------------------------ */
 

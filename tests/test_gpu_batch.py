@@ -1,0 +1,125 @@
+"""GPU parity of the batched path: every game of a mixed batch equals the oracle's least
+fixpoint and the single-game solver's result on the same game (alone == in the mix)."""
+import numpy as np
+import pytest
+
+from conftest import lfp, to_oracle_game
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def _mixed_games(seed, count, nmax):
+    import egsolver_amd as E
+    rng = np.random.default_rng(seed)
+    games = []
+    for i in range(count):
+        kind = i % 5
+        n = int(rng.integers(5, nmax))
+        if kind == 4:       # complete digraph, as the batched family's every 16th game
+            p = E.GenParams(n=min(n, 120), seed=seed * 1000 + i, deg_kind=3, deg_lo=1, deg_hi=1,
+                            wmax=int(rng.integers(1, 60)))
+        elif kind == 3:     # skewed degrees
+            p = E.GenParams(n=n, seed=seed * 1000 + i, deg_kind=2, deg_lo=1, deg_hi=min(n - 1, 400),
+                            alpha=1.7, wmax=int(rng.integers(1, 300)))
+        else:
+            p = E.GenParams(n=n, seed=seed * 1000 + i, deg_kind=1, deg_lo=2, deg_hi=min(n - 1, 16),
+                            wmax=int(rng.integers(10, 1000)))
+        games.append(E.generate(p))
+    return games
+
+
+def test_batch_matches_oracle_and_alone(oracle):
+    import egsolver_amd as E
+    games = _mixed_games(1, 40, 900)
+    _say(f"batch of {len(games)} games, sizes {min(g.n for g in games)}..{max(g.n for g in games)}")
+    refs = [lfp(oracle, to_oracle_game(g)) for g in games]
+    with E.Batch() as b:
+        for g in games:
+            b.add(g)
+        b.upload()
+        for opt in [E.batch_options(), E.batch_options(threads=256, lds_ints=64),
+                    E.batch_options(threads=1024, blocks_per_cu=1, lanes_per_edge_x16=16),
+                    E.batch_options(group_max=1), E.batch_options(group_min=64)]:
+            st = b.solve(opt)
+            assert st["converged"] == len(games)
+            vals = b.all_values()
+            for i, (g, ref) in enumerate(zip(games, refs)):
+                assert np.array_equal(vals[i], ref), (i, g.n, np.count_nonzero(vals[i] != ref))
+    for i in (0, 7, 19, 33):
+        alone, _ = E.solve(games[i])
+        assert np.array_equal(alone, refs[i])
+
+
+def test_batch_unpacked_edges(oracle):
+    """Weights too wide for 32-bit packing take the 64-bit edge path."""
+    import egsolver_amd as E
+    games = _mixed_games(2, 12, 400)
+    big = E.GenParams(n=300, seed=77, deg_kind=1, deg_lo=2, deg_hi=9, wmax=6_000_000)
+    games.append(E.generate(big))
+    _say("batch unpacked")
+    with E.Batch() as b:
+        for g in games:
+            b.add(g)
+        b.upload()
+        st = b.solve()
+        assert st["packed_edges"] == 0 and st["converged"] == len(games)
+        for i, g in enumerate(games):
+            assert np.array_equal(b.values(i), lfp(oracle, to_oracle_game(g))), i
+
+
+def test_batch_config_family_sample(oracle):
+    """A sample of the batched workload's own family (sizes scaled down 10x)."""
+    import egsolver_amd as E
+    games = []
+    for i in range(24):
+        p = E.family_params(E.games.FAMILY_BATCH, 4, i)
+        p.n = max(50, p.n // 10) if p.deg_kind != 3 else p.n
+        games.append(E.generate(p, stream=i))
+    _say(f"family sample {[g.n for g in games]}")
+    with E.Batch() as b:
+        for g in games:
+            b.add(g)
+        b.upload()
+        st = b.solve()
+        assert st["converged"] == len(games)
+        for i, g in enumerate(games):
+            assert np.array_equal(b.values(i), lfp(oracle, to_oracle_game(g))), i
+
+
+def test_batch_sweep_cap_is_partial(oracle):
+    import egsolver_amd as E
+    games = _mixed_games(3, 6, 600)
+    with E.Batch() as b:
+        for g in games:
+            b.add(g)
+        b.upload()
+        st = b.solve(E.batch_options(max_sweeps=2))
+        for i, g in enumerate(games):
+            ref = lfp(oracle, to_oracle_game(g))
+            assert np.all(b.values(i) <= ref)
+            assert b.game_stats(i)["sweeps"] <= 2
+
+
+def test_batch_with_and_without_proofs(oracle):
+    """In-workgroup TOP proofs (frequent, and switched off) leave every game exact."""
+    import egsolver_amd as E
+    games = _mixed_games(5, 30, 2500)
+    games.append(E.config_game(2, scale=1 / 1024))
+    _say("batch proofs on/off")
+    refs = [lfp(oracle, to_oracle_game(g)) for g in games]
+    with E.Batch() as b:
+        for g in games:
+            b.add(g)
+        b.upload()
+        for every in (2, 7, -1):
+            st = b.solve(E.batch_options(certify_every=every))
+            assert st["converged"] == len(games)
+            if every > 0:
+                assert st["certificates"] >= 1
+            vals = b.all_values()
+            for i, ref in enumerate(refs):
+                assert np.array_equal(vals[i], ref), (every, i, np.count_nonzero(vals[i] != ref))

@@ -1,0 +1,67 @@
+"""GPU parity of the serving loop: every request's result equals the oracle's least
+fixpoint and the single-game solver's result on the same game, whatever mix of other
+requests shared its slices (bar: bit-exact)."""
+import numpy as np
+import pytest
+
+from conftest import lfp, to_oracle_game
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def _requests(seed, count, nmax, spacing):
+    import egsolver_amd as E
+    rng = np.random.default_rng(seed)
+    reqs = []
+    t = 0.0
+    for i in range(count):
+        p = E.family_params(E.games.FAMILY_SERVE, seed, i)
+        if p.deg_kind != 3:
+            p.n = int(min(p.n, nmax))
+        reqs.append(E.Request(rid=i, arrival=t, game=E.generate(p, stream=i)))
+        t += float(rng.exponential(spacing))
+    return reqs
+
+
+def test_served_equals_alone_and_oracle(oracle):
+    import egsolver_amd as E
+    reqs = _requests(5, 24, 3000, 0.002)
+    _say(f"serve {len(reqs)} requests, sizes {[r.game.n for r in reqs]}")
+    # a small arena forces admission waits and slot reuse
+    arena = sum(E.Server.footprint(r.game) for r in reqs) // 3
+    done, wall, slices = E.run_served(reqs, arena, E.serve_options(slice_sweeps=8), max_games=16, keep_values=True)
+    assert all(r.finished >= r.arrival for r in done)
+    assert slices >= 2
+    for r in done:
+        ref = lfp(oracle, to_oracle_game(r.game))
+        assert np.array_equal(r.values, ref), (r.rid, r.game.n)
+        alone, _ = E.solve(r.game)
+        assert np.array_equal(alone, ref)
+
+
+def test_server_capacity_and_reuse():
+    import egsolver_amd as E
+    reqs = _requests(6, 6, 1500, 0.0)
+    need = max(E.Server.footprint(r.game) for r in reqs)
+    _say("serve capacity")
+    with E.Server(need + 4096, max_games=4) as srv:
+        t0 = srv.submit(reqs[0].game)
+        assert t0 is not None
+        # a game larger than what is left is refused until the first one leaves
+        if E.Server.footprint(reqs[1].game) > srv.free_bytes():
+            assert srv.submit(reqs[1].game) is None
+        else:
+            t_extra = srv.submit(reqs[1].game)
+            assert t_extra is not None
+        for _ in range(10000):
+            srv.launch(E.serve_options(slice_sweeps=4))
+            if t0 in srv.wait():
+                break
+        vals, st = srv.fetch(t0)
+        assert st["sweeps"] >= 1
+        t1 = srv.submit(reqs[1].game)
+        assert t1 is not None
