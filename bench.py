@@ -1,20 +1,25 @@
-"""Benchmark of the energy-game fixpoint on one MI355X.
+"""Benchmark of the energy-game solver on one MI355X.
 
-Workload (default): the single large uniform game -- n = 2^22 vertices, owner uniform,
-out-degree exactly 8 distinct uniform successors (m = 2^25), weights uniform in
-[-100, 100], seed 2.  The game is generated, loaded into HBM and transposed before the
-timed region.  One step is one fixpoint round: one pass of the lift kernel over the
-current frontier, with in-place value updates and the predecessor worklist rebuild.
-`value` is edges relaxed per second over the timed rounds.
+Metric (one JSON line on stdout): edges relaxed per second to fixpoint on the single large
+uniform game -- n = 2^22 vertices, owner uniform, out-degree exactly 8 distinct uniform
+successors (m = 2^25), weights uniform in [-100, 100], seed 2.  The game is generated,
+copied into HBM and loaded into the solver before the timed region.  One step is one full
+solve to the least fixpoint: transpose, initial frontier, rounds and TOP proofs.  `value`
+= edges relaxed by the solves / their wall time.
 
-The JSON line also carries
-  roofline      the dominant kernel (the lift kernel) against MI355X's 8 TB/s HBM peak:
-                algorithmic bytes per launch / average launch time (HIP events on the
-                solver's stream), plus the device's measured copy bandwidth and, when a
-                profile of this workload is committed, the profiler-measured HBM traffic.
-  cpu_baseline  the reference's own CPU solver (cpu_solver.c compiled from its sources
-                into oracle/_ref; the oracle's C restatement when that build is absent)
-                on a bounded sample of the same game, timed on this host.
+The line also carries
+  roofline      the lift kernel on plain dense rounds of the same game (proofs off, so every
+                round relaxes all m edges): algorithmic bytes per launch / average launch
+                time (HIP events on the solver's stream) against the 8 TB/s HBM peak, the
+                measured copy-kernel bandwidth, and the profiler's HBM bytes per launch
+                from the committed counter summary of this workload.
+  cpu_baseline  the reference's own CPU solver (cpu_solver.c compiled from its sources into
+                oracle/_ref; the oracle's restatement if that build is absent), EG0 Jacobi
+                sweeps on this host, on a bounded number of sweeps of the same game.
+  batched       4096 games of the batched family (seed 4) in one launch vs one game per
+                launch (single-game path, inputs resident) on a stratified sample.
+  serving       a Poisson request stream of the serving family (seed 5) through the
+                continuous-batching server vs one game per launch, p50/p99 latency.
 """
 from __future__ import annotations
 
@@ -24,6 +29,9 @@ import os
 import re
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -32,31 +40,31 @@ METRIC = "edges relaxed/s to fixpoint; solves/s + p99 latency, batched vs plain"
 HBM_PEAK_GBPS = 8000.0
 
 
-def algorithmic_bytes(st_delta):
-    """B = sum_{processed v} (16 + 12 d_out(v)) + sum_{lifted v} (12 + 12 d_in(v)):
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(d):
+    """B = sum_{processed v} (16 + 12 d_out(v)) + sum_{risen v} (12 + 12 d_in(v)):
     frontier id 4 + row offsets 8 + own value 4; per out-edge successor 4 + weight 4 +
-    gathered value 4; per lifted vertex value store 4 + CSC offsets 8; per in-edge of a
-    lifted vertex predecessor id 4 + dedupe stamp read-modify-write 8."""
-    return (16 * st_delta["processed"] + 12 * st_delta["edges_relaxed"] + 12 * st_delta["lifted"]
-            + 12 * st_delta["pred_visits"])
+    gathered value 4; per risen vertex value store 4 + CSC offsets 8; per in-edge of a
+    risen vertex predecessor id 4 + dedupe stamp read-modify-write 8."""
+    return 16 * d["processed"] + 12 * d["edges_relaxed"] + 12 * d["lifted"] + 12 * d["pred_visits"]
 
 
 def read_profile_traffic(workload_key):
-    """HBM bytes per launch of the lift kernel from the committed rocprofv3 counter
-    summary of this workload (profiles/lift_<workload>_pmc.md), or None."""
+    """HBM bytes per launch of the lift kernel from profiles/lift_<workload>_pmc.md."""
     path = os.path.join(ROOT, "profiles", f"lift_{workload_key}_pmc.md")
     if not os.path.exists(path):
         return None
-    txt = open(path).read()
-    mo = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    mo = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
     return float(mo.group(1)) if mo else None
 
 
 def cpu_baseline(g, budget_s=20.0):
-    """Time the reference CPU solver on a bounded sample of the same game: EG0 Jacobi
-    sweeps (cpu_solver.c:109-185; every sweep relaxes all m edges) on all the host
-    threads OpenMP is given, for a fixed sweep count sized to ~budget_s."""
-    from oracle.oracle import Reference, Oracle, game_from_arrays
+    """The reference CPU solver on a bounded sample of the same game: EG0 Jacobi sweeps
+    (cpu_solver.c:109-185; a sweep relaxes all m edges), OpenMP guided(4)."""
+    from oracle.oracle import Oracle, Reference, game_from_arrays
 
     og = game_from_arrays(g.n, g.n0, g.ptr, g.succ, g.w)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
@@ -77,68 +85,231 @@ def cpu_baseline(g, budget_s=20.0):
     sweeps = max(2, int(budget_s / max(probe / 2, 1e-3)))
     secs = run(sweeps)
     return {"value": sweeps * g.m / secs, "unit": "edges relaxed/s", "cores": threads, "kind": kind,
-            "sample": f"EG0 (--cpu --eg0 {sweeps}): {sweeps} Jacobi sweeps over all {g.m} edges of the same "
-                      f"game from f=0, {secs:.2f} s, OpenMP guided(4), {threads} threads"}
+            "sample": f"EG0 (--cpu --eg0 {sweeps}): {sweeps} Jacobi sweeps over all {g.m} edges of the same game "
+                      f"from f=0, {secs:.2f} s, OpenMP guided(4), {threads} threads"}
+
+
+def single_game(E, g, args):
+    """Headline: full solves to fixpoint; plus plain dense rounds for the roofline."""
+    s = E.Solver(g.n, g.m)
+    s.load(g)
+    shape = {"group": args.group, "grid": args.grid}
+    if args.tune:
+        from egsolver_amd.autotune import tune_single
+        best, table = tune_single(g, solver=s)
+        shape.update({k: best[k] for k in ("group", "grid")})
+        log(f"tuned single-game shape {shape} over {len(table)} candidates")
+    # plain rounds (no proofs): the kernel measurement for the roofline
+    s.prepare(E.options(group=shape["group"], grid=shape["grid"], certify_every=-1))
+    _, a = s.iterate(args.warmup)
+    _, b = s.iterate(args.rounds)
+    d = {k: getattr(b, k) - getattr(a, k) for k in ("processed", "edges_relaxed", "lifted", "pred_visits", "phases")}
+    ms_dev = b.ms_iterate - a.ms_iterate
+    rounds = d["phases"]
+    per_launch_ms = ms_dev / max(rounds, 1)
+    abytes = algorithmic_bytes(d) / max(rounds, 1)
+    achieved = abytes / (per_launch_ms * 1e-3) / 1e9
+    plain = {"rounds": rounds, "device_ms_per_round": per_launch_ms,
+             "edges_per_s": d["edges_relaxed"] / (ms_dev * 1e-3), "algorithmic_bytes_per_round": abytes,
+             "achieved_gbps": achieved}
+    log(f"plain rounds: {per_launch_ms:.3f} ms/round, {plain['edges_per_s']:.3e} edges/s, {achieved:.0f} GB/s")
+    # full solves to fixpoint (proofs on)
+    opt = E.options(group=shape["group"], grid=shape["grid"])
+    for _ in range(args.warmup_solves):
+        s.prepare(opt)
+        s.iterate()
+    solves = []
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        s.prepare(opt)
+        conv, st = s.iterate()
+        solves.append((time.perf_counter() - t0, st))
+        if not conv:
+            raise SystemExit("a solve did not reach the fixpoint")
+    t_all = time.perf_counter() - t_all
+    vals = s.values()
+    violations = s.violations()
+    s.close()
+    if violations:
+        raise SystemExit(f"fixpoint certificate failed: {violations} vertices")
+    edges = sum(st.edges_relaxed for _, st in solves)
+    last = solves[-1][1]
+    info = {"solve_ms_mean": 1e3 * t_all / args.steps, "solve_ms_min": 1e3 * min(t for t, _ in solves),
+            "rounds_per_solve": last.phases, "proofs": last.certificates, "proof_passes": last.cert_passes,
+            "top_marked": last.top_marked, "top_vertices": int((vals == 2147483647).sum()),
+            "value_checksum": int(np.bitwise_xor.reduce(vals.astype(np.uint32) * np.uint32(2654435761))),
+            "fixpoint_violations": violations, "device_ms_prepare": last.ms_prepare,
+            "device_ms_iterate": last.ms_iterate}
+    return edges / t_all, t_all, info, plain, shape
+
+
+def gen_many(E, family, seed, count, scale_n=1.0, workers=16):
+    def one(i):
+        p = E.family_params(family, seed, i)
+        if scale_n != 1.0 and p.deg_kind != 3:
+            p.n = max(50, int(p.n * scale_n))
+        return E.generate(p, stream=i)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(one, range(count)))
+
+
+def plain_solve_sample(E, games, check_against=None):
+    """One game per launch (single-game path, proofs on): per game load (untimed), then
+    prepare + iterate + read back (timed)."""
+    nmax = max(g.n for g in games)
+    mmax = max(g.m for g in games)
+    s = E.Solver(nmax, mmax)
+    t = 0.0
+    try:
+        for i, g in enumerate(games):
+            s.load(g)
+            t0 = time.perf_counter()
+            s.prepare()
+            s.iterate()
+            v = s.values()
+            t += time.perf_counter() - t0
+            if check_against is not None and not np.array_equal(v, check_against[i]):
+                raise SystemExit(f"plain and batched results differ on game {i}")
+    finally:
+        s.close()
+    return t
+
+
+def batched(E, args):
+    from egsolver_amd.autotune import tune_batch
+    t0 = time.perf_counter()
+    games = gen_many(E, E.games.FAMILY_BATCH, 4, args.batch_games)
+    nv = sum(g.n for g in games)
+    ne = sum(g.m for g in games)
+    log(f"batched: {len(games)} games, {nv} vertices, {ne} edges generated in {time.perf_counter() - t0:.1f}s")
+    opt = E.batch_options()
+    tuned = None
+    if args.tune:
+        opt, table = tune_batch(games, sample=48)
+        tuned = {k: getattr(opt, k) for k in ("threads", "lds_ints", "blocks_per_cu", "lanes_per_edge_x16")}
+        log(f"tuned batch options {tuned} over {len(table)} candidates")
+    with E.Batch() as b:
+        for g in games:
+            b.add(g)
+        b.upload()
+        b.solve(opt)                      # warm-up launch
+        t1 = time.perf_counter()
+        st = b.solve(opt)
+        vals = b.all_values()
+        wall = time.perf_counter() - t1
+    if st["converged"] != len(games):
+        raise SystemExit("batched: not every game converged")
+    # plain sample: spread over the size distribution (sorted by n, every len/k-th)
+    order = np.argsort([g.n for g in games], kind="stable")
+    idx = order[np.linspace(0, len(games) - 1, min(args.plain_games, len(games))).round().astype(int)]
+    sub = [games[i] for i in idx]
+    t_plain = plain_solve_sample(E, sub, check_against=[vals[i] for i in idx])
+    plain_rate = len(sub) / t_plain
+    out = {"workload": f"{len(games)} games of the batched family (n log-uniform 1e3..5e4, out-degree 2..16, "
+                       f"W log-uniform 10..1000, every 16th a complete digraph of 200..500), seed 4",
+           "vertices": nv, "edges": ne, "solves_per_s": len(games) / wall, "batch_wall_s": wall,
+           "batch_device_ms": st["ms"], "sweeps": st["sweeps"], "proofs": st["certificates"],
+           "edges_relaxed_per_s": st["edges_relaxed"] / (st["ms"] * 1e-3),
+           "plain_solves_per_s": plain_rate, "plain_sample_games": len(sub),
+           "speedup_vs_plain": (len(games) / wall) / plain_rate, "options": tuned,
+           "exact_vs_plain_on_sample": True}
+    log(f"batched: {out['solves_per_s']:.1f} solves/s vs plain {plain_rate:.1f} solves/s")
+    return out, games
+
+
+def serving(E, args):
+    from egsolver_amd.serve import Request, latency_stats, run_plain, run_served, serve_options
+    R = args.serve_requests
+    t0 = time.perf_counter()
+    games = gen_many(E, E.games.FAMILY_SERVE, 5, R)
+    log(f"serving: {R} requests generated in {time.perf_counter() - t0:.1f}s")
+    # capacity of the batched kernel on this distribution, to set the arrival rate
+    with E.Batch() as b:
+        for g in games[: min(R, 128)]:
+            b.add(g)
+        b.upload()
+        st = b.solve()
+        cap = min(R, 128) / (st["ms"] * 1e-3)
+    rate = args.serve_load * cap
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=R))
+    arrivals -= arrivals[0]
+    reqs = [Request(rid=i, arrival=float(arrivals[i]), game=games[i]) for i in range(R)]
+    arena = int(args.serve_arena_gb * (1 << 30))
+    opt = serve_options(slice_sweeps=args.slice_sweeps)
+    done, wall, slices = run_served(reqs, arena, opt, max_games=4096)
+    served = latency_stats(done)
+    served.update({"solves_per_s": R / wall, "wall_s": wall, "slices": slices})
+    # the same stream, one game per launch (first requests only: the plain path saturates)
+    P = min(R, args.serve_plain_requests)
+    nmax = max(g.n for g in games[:P])
+    mmax = max(g.m for g in games[:P])
+    solver = E.Solver(nmax, mmax)
+
+    def one(g):
+        solver.load(g)
+        solver.prepare()
+        solver.iterate()
+        return solver.values()
+    preqs = [Request(rid=i, arrival=float(arrivals[i]), game=games[i]) for i in range(P)]
+    pdone, pwall = run_plain(preqs, one)
+    solver.close()
+    plain = latency_stats(pdone)
+    plain.update({"solves_per_s": P / pwall, "requests": P})
+    log(f"serving: served p50 {served['p50_ms']:.1f} ms p99 {served['p99_ms']:.1f} ms {served['solves_per_s']:.1f}/s; "
+        f"plain p50 {plain['p50_ms']:.1f} p99 {plain['p99_ms']:.1f} {plain['solves_per_s']:.1f}/s")
+    return {"workload": f"{R} requests of the serving family (n log-uniform 1e3..5e5, otherwise as batched), "
+                        f"Poisson arrivals at {rate:.1f}/s = {args.serve_load:.2f} x measured batched capacity, seed 5",
+            "served": served, "plain_one_game_per_launch": plain, "arrival_rate_per_s": rate,
+            "arena_gb": args.serve_arena_gb}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=5, help="timed full solves of the large game")
+    ap.add_argument("--warmup", type=int, default=50, help="untimed plain rounds before the roofline rounds")
+    ap.add_argument("--warmup-solves", type=int, default=1)
+    ap.add_argument("--rounds", type=int, default=600, help="plain rounds timed for the roofline")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink n (testing only)")
     ap.add_argument("--group", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--tune", action="store_true", help="autotune launch shapes before timing")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-games", type=int, default=4096)
+    ap.add_argument("--plain-games", type=int, default=128)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-plain-requests", type=int, default=150)
+    ap.add_argument("--serve-load", type=float, default=0.75)
+    ap.add_argument("--serve-arena-gb", type=float, default=24.0)
+    ap.add_argument("--slice-sweeps", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(the single-game path shards by vertex range, see the design notes).", file=sys.stderr)
+              f"(how the work would shard across GPUs is described in the design notes).", file=sys.stderr)
         sys.exit(2)
 
     import egsolver_amd as E
 
     t0 = time.time()
     g = E.config_game(2, scale=args.scale)
-    gen_s = time.time() - t0
-    print(f"[bench] game n={g.n} m={g.m} mg={g.mg} generated in {gen_s:.1f}s", file=sys.stderr, flush=True)
-
-    copy_gbps = _copy_bw(E)
-
-    s = E.Solver(g.n, g.m)
-    s.load(g)
-    opt = E.options(group=args.group, grid=args.grid)
-    s.prepare(opt)
-    prep = s.stats()
-    print(f"[bench] prepared (transpose+init) in {prep.ms_prepare:.2f} ms; initial frontier "
-          f"{prep.initial_active}", file=sys.stderr, flush=True)
-
-    conv, st_w = s.iterate(args.warmup)
-    w_stats = st_w.as_dict()
-    t_wall = time.perf_counter()
-    conv, st_t = s.iterate(args.steps)
-    t_wall = time.perf_counter() - t_wall
-    t_stats = st_t.as_dict()
-    delta = {k: t_stats[k] - w_stats[k] for k in ("processed", "edges_relaxed", "lifted", "pred_visits", "phases")}
-    ms_dev = t_stats["ms_iterate"] - w_stats["ms_iterate"]
-    rounds = delta["phases"]
-    if rounds <= 0:
-        raise SystemExit("no rounds ran in the timed region (game converged during warm-up)")
-    edges_per_s = delta["edges_relaxed"] / t_wall
-    abytes = algorithmic_bytes(delta)
-    per_launch_ms = ms_dev / rounds
-    achieved = abytes / rounds / (per_launch_ms * 1e-3) / 1e9
+    log(f"game n={g.n} m={g.m} mg={g.mg} generated in {time.time() - t0:.1f}s")
+    copy_gbps = _copy_bw()
+    value, t_all, info, plain, shape = single_game(E, g, args)
     traffic = read_profile_traffic("config2")
     line = {
         "metric": METRIC,
-        "value": edges_per_s,
+        "value": value,
         "unit": "edges relaxed/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": t_wall * 1e3 / args.steps,
+        "warmup": args.warmup_solves,
+        "ms_per_step": 1e3 * t_all / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -146,41 +317,29 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "single large uniform game (n=2^22, out-degree 8, weights U[-100,100], seed 2); "
-                        "step = one fixpoint round over the frontier",
-            "n": g.n, "m": g.m, "mg": g.mg, "rounds_timed": rounds, "rounds_before": w_stats["phases"],
-            "group": s_shape_group(args.group, g), "prepare_ms": prep.ms_prepare,
-            "edges_per_round": delta["edges_relaxed"] / rounds,
-            "lifted_per_round": delta["lifted"] / rounds,
-            "device_ms_per_round": per_launch_ms,
+                        "step = one full solve to the least fixpoint (transpose, rounds, TOP proofs)",
+            "n": g.n, "m": g.m, "mg": g.mg, "shape": shape, **info,
+            "plain_rounds": plain,
         },
         "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": traffic,
-            "kernel": f"k_round<{s_shape_group(args.group, g)}>",
-            "algorithmic_bytes_per_launch": abytes / rounds,
+            "bound": "hbm", "achieved": plain["achieved_gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": plain["achieved_gbps"] / HBM_PEAK_GBPS, "traffic": traffic,
+            "kernel": "k_round (dense sweep)",
+            "algorithmic_bytes_per_launch": plain["algorithmic_bytes_per_round"],
+            "avg_launch_ms": plain["device_ms_per_round"],
             "copy_kernel_gbps": copy_gbps,
         },
     }
-    s.close()
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(g, args.cpu_budget)
-    else:
-        line["cpu_baseline"] = None
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(g, args.cpu_budget)
+    del g
+    if not args.no_batch:
+        line["batched"], _ = batched(E, args)
+    if not args.no_serve:
+        line["serving"] = serving(E, args)
     print(json.dumps(line), flush=True)
 
 
-def s_shape_group(group, g):
-    if group:
-        return group
-    avg = g.m / g.n
-    p = 1
-    while p < avg:
-        p <<= 1
-    return min(16, p)
-
-
-def _copy_bw(E):
+def _copy_bw():
     import ctypes
     from egsolver_amd import _native as N
     out = ctypes.c_double(0)
