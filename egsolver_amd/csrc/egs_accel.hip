@@ -19,6 +19,10 @@
 //   4. Then every play from D under sigma either reaches a TOP vertex or loses energy
 //      without bound, so lfp = TOP on D; raising f to TOP there keeps f <= lfp, and the
 //      ordinary rounds continue to the same least fixpoint.
+// Steps 2-3 are checked, not assumed, so the choice of B only decides which D is tried:
+// an attempt walks the admissible thresholds upwards until one D passes both checks.
+#include <algorithm>
+
 #include "egs_device.h"
 #include "egs_internal.h"
 
@@ -134,24 +138,27 @@ hipError_t launch_abs_wmax(const DevGame &g, int *out, hipStream_t st) {
     return hipGetLastError();
 }
 
-// Pick the smallest admissible threshold from the two histograms (host side).
-static int64_t pick_threshold(const int *win, const int *flo, int wmax, int mg) {
-    int64_t above = 0;
+// Admissible thresholds, smallest first (host side): B_j = W * 2^j with an empty window,
+// some non-TOP value above it and B_j <= MG - W.  Returns how many were written.
+int gap_candidates(const int *win, const int *flo, int wmax, int mg, int64_t *out, int max_out) {
     int64_t suffix[kBins + 1];
     suffix[kBins] = 0;
     for (int k = kBins - 1; k >= 0; k--) suffix[k] = suffix[k + 1] + flo[k];
-    for (int j = 1; j < 40; j++) {
+    int cnt = 0;
+    for (int j = 1; j < 40 && cnt < max_out; j++) {
         const int64_t B = (int64_t)wmax << j;
         if (B > (int64_t)mg - wmax) break;
-        above = suffix[j];
-        if (win[j] == 0 && above > 0) return B;
+        if (win[j] == 0 && suffix[j] > 0) out[cnt++] = B;
     }
-    return -1;
+    return cnt;
 }
 
-// One certification attempt on a paused run.  Returns 1 when vertices were set to TOP
-// (count in *marked), 0 when no proof was found, < 0 on a HIP error.
-int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int max_passes, hipStream_t st,
+// One certification attempt on a paused run.  Each admissible threshold, smallest first,
+// gets the closure check and at most `cap` longest-path passes; a smaller-D candidate is
+// tried when a larger D fails (a finite vertex parked above the lowest gap keeps that D
+// from ever closing).  Returns 1 when vertices were set to TOP (count in res.marked), 0
+// when no candidate gave a proof, < 0 on a HIP error.
+int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int cap, int max_cands, hipStream_t st,
                 CertResult &res) {
     res = CertResult{};
     if (wmax <= 0) return 0;
@@ -160,45 +167,50 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int max_passe
     hipLaunchKernelGGL(k_gap_hist, dim3(grid), dim3(256), 0, st, g, wmax, cb.hist);
     if (hipMemcpyAsync(cb.pinned, cb.hist, sizeof(int) * 2 * kBins, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
     if (hipStreamSynchronize(st) != hipSuccess) return -1;
-    const int64_t B64 = pick_threshold(cb.pinned, cb.pinned + kBins, wmax, g.mg);
-    if (B64 < 0) return 0;
-    const int B = (int)B64;
-    res.threshold = B;
+    int64_t cands[40];
+    const int ncand = gap_candidates(cb.pinned, cb.pinned + kBins, wmax, g.mg, cands, std::min(max_cands, 40));
     int *flags = cb.hist + 2 * kBins;       // [0] closure failed, [1] bound exceeded, [2] marked
-    hipLaunchKernelGGL(k_sigma, dim3(grid), dim3(256), 0, st, g, B, cb.sigma, cb.h, flags);
+    int *cnt = cb.pinned + 2 * kBins;       // kChunk pass counters
+    int *fl = cnt + 8;                      // flags readback
     const long long bound = (long long)g.n * ((long long)(g.n + 1) * wmax + 1);
     constexpr int kChunk = 8;
-    int passes = 0;
-    bool converged = false, failed = false;
-    while (!converged && !failed && passes < max_passes) {
-        if (hipMemsetAsync(cb.bfcnt, 0, sizeof(int) * kChunk, st) != hipSuccess) return -1;
-        for (int k = 0; k < kChunk; k++)
-            hipLaunchKernelGGL(k_bf_pass, dim3(grid), dim3(256), 0, st, g, B, cb.sigma, cb.h, bound, cb.bfcnt + k,
-                               flags);
-        if (hipMemcpyAsync(cb.pinned + 2 * kBins, cb.bfcnt, sizeof(int) * kChunk, hipMemcpyDeviceToHost, st) != hipSuccess)
-            return -1;
-        if (hipMemcpyAsync(cb.pinned + 2 * kBins + kChunk, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st) != hipSuccess)
-            return -1;
+    for (int c = 0; c < ncand; c++) {
+        const int B = (int)cands[c];
+        res.threshold = B;
+        res.tried++;
+        if (hipMemsetAsync(flags, 0, sizeof(int) * 3, st) != hipSuccess) return -1;
+        hipLaunchKernelGGL(k_sigma, dim3(grid), dim3(256), 0, st, g, B, cb.sigma, cb.h, flags);
+        if (hipMemcpyAsync(fl, flags, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
         if (hipStreamSynchronize(st) != hipSuccess) return -1;
-        const int *cnt = cb.pinned + 2 * kBins;
-        const int *fl = cb.pinned + 2 * kBins + kChunk;
-        if (fl[0] || fl[1]) { failed = true; break; }
-        for (int k = 0; k < kChunk; k++) {
-            passes++;
-            if (cnt[k] == 0) { converged = true; break; }
+        if (fl[0]) { res.closure_failed++; continue; }
+        int passes = 0;
+        bool converged = false, diverged = false;
+        while (!converged && !diverged && passes < cap) {
+            if (hipMemsetAsync(cb.bfcnt, 0, sizeof(int) * kChunk, st) != hipSuccess) return -1;
+            for (int k = 0; k < kChunk; k++)
+                hipLaunchKernelGGL(k_bf_pass, dim3(grid), dim3(256), 0, st, g, B, cb.sigma, cb.h, bound,
+                                   cb.bfcnt + k, flags);
+            if (hipMemcpyAsync(cnt, cb.bfcnt, sizeof(int) * kChunk, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+            if (hipMemcpyAsync(fl, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+            if (hipStreamSynchronize(st) != hipSuccess) return -1;
+            if (fl[1]) diverged = true;
+            for (int k = 0; k < kChunk && !diverged; k++) {
+                passes++;
+                if (cnt[k] == 0) { converged = true; break; }
+            }
         }
+        res.passes += passes;
+        if (!converged) {
+            if (!diverged) res.cap_hit = 1;
+            continue;
+        }
+        hipLaunchKernelGGL(k_mark_top, dim3(grid), dim3(256), 0, st, g, B, flags + 2);
+        if (hipMemcpyAsync(fl + 2, flags + 2, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+        if (hipStreamSynchronize(st) != hipSuccess) return -1;
+        res.marked = fl[2];
+        return res.marked > 0 ? 1 : 0;
     }
-    res.passes = passes;
-    if (!converged || failed) {
-        res.closure_failed = cb.pinned[2 * kBins + kChunk];
-        return 0;
-    }
-    hipLaunchKernelGGL(k_mark_top, dim3(grid), dim3(256), 0, st, g, B, flags + 2);
-    if (hipMemcpyAsync(cb.pinned + 2 * kBins + kChunk + 2, flags + 2, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
-        return -1;
-    if (hipStreamSynchronize(st) != hipSuccess) return -1;
-    res.marked = cb.pinned[2 * kBins + kChunk + 2];
-    return res.marked > 0 ? 1 : 0;
+    return 0;
 }
 
 }  // namespace egs

@@ -14,6 +14,7 @@
 #include "../../include/egs.h"
 #include "egs_device.h"
 #include "egs_internal.h"
+#include "egs_pack.h"
 
 namespace {
 
@@ -64,6 +65,8 @@ constexpr int kPinStats = 640;    // 4 x u64
 constexpr int kPinViol = 704;     // 1 x u64
 constexpr int kPinCert = 1024;    // certification readbacks (2 x 64 + 16 ints)
 constexpr int kPinForce = 1280;   // round number forced to a full sweep
+constexpr int kCertFirstCap = 64;     // longest-path passes per candidate at the first attempt
+constexpr int kCertCandidates = 8;    // thresholds tried per attempt
 
 int pow2ceil(int x) {
     int p = 1;
@@ -89,6 +92,7 @@ struct egs_solver {
     long long *d_h = nullptr;
     int wmax = 0;
     int cert_every = 32, cert_gap = 32, cert_max_passes = 1024;
+    int cert_cap = 64;        // longest-path passes per candidate; doubles while attempts run out of passes
     int64_t next_cert = 0;
     int2 *d_edges = nullptr;
     unsigned long long *d_stats = nullptr, *d_statsum = nullptr;
@@ -271,23 +275,21 @@ int egs_solver_load(egs_solver *s, int mem, int32_t n, int32_t n0, int64_t m, in
     HIP_TRY(hipSetDevice(s->device));
     if (mem == EGS_MEM_HOST) {
         // validate the CSR on the host before anything reaches the device
-        if (ptr[0] != 0 || ptr[n] != m)
+        int32_t wm = 0, mx = 0;
+        int64_t bad = -1;
+        switch (egs_pk_check(n, m, ptr, succ, w, &wm, &mx, &bad)) {
+        case EGS_PK_BAD_OFFSETS:
             return fail(EGS_E_ARG, "CSR offsets must run from 0 to m (got %d..%d)", ptr[0], ptr[n]);
-        int mx = 0;
-        for (int v = 0; v < n; v++) {
-            int d = ptr[v + 1] - ptr[v];
-            if (d < 1) return fail(EGS_E_ARG, "vertex %d has no out-edge", v);
-            mx = std::max(mx, d);
+        case EGS_PK_NO_EDGE: return fail(EGS_E_ARG, "vertex %lld has no out-edge", (long long)bad);
+        case EGS_PK_BAD_SUCC:
+            return fail(EGS_E_ARG, "edge %lld: successor %d out of range", (long long)bad, succ[bad]);
+        default: break;
         }
-        for (int64_t e = 0; e < m; e++)
-            if (succ[e] < 0 || succ[e] >= n) return fail(EGS_E_ARG, "edge %lld: successor %d out of range", (long long)e, succ[e]);
         s->max_outdeg = mx;
-        int wm = 0;
-        for (int64_t e = 0; e < m; e++) wm = std::max(wm, w[e] < 0 ? -w[e] : w[e]);
         s->wmax = wm;
         int2 *packed = (int2 *)malloc(sizeof(int2) * (size_t)m);
         if (!packed) return fail(EGS_E_NOMEM, "host staging allocation failed");
-        for (int64_t e = 0; e < m; e++) packed[e] = make_int2(succ[e], w[e]);
+        egs_pk_pack64(m, succ, w, (int32_t *)packed);
         hipError_t e1 = hipMemcpy(s->d_ptr, ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice);
         hipError_t e2 = hipMemcpy(s->d_edges, packed, sizeof(int2) * (size_t)m, hipMemcpyHostToDevice);
         free(packed);
@@ -393,6 +395,7 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     s->next_round = 1;
     s->cert_gap = s->cert_every;
     s->next_cert = s->cert_every;
+    s->cert_cap = std::min(kCertFirstCap, s->cert_max_passes);
     s->prepared = true;
     return EGS_OK;
 }
@@ -436,7 +439,7 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
             if (list_total(s, s->h_probe + kPinConv) == 0) break;
             egs::CertBuffers cb{s->d_hist, s->d_sigma, s->d_h, s->d_bfcnt, s->h_probe + kPinCert};
             egs::CertResult cr;
-            const int got = egs::certify_top(s->g, cb, s->wmax, s->cert_max_passes, s->stream, cr);
+            const int got = egs::certify_top(s->g, cb, s->wmax, s->cert_cap, kCertCandidates, s->stream, cr);
             if (got < 0) return fail(EGS_E_HIP, "certification: %s", hipGetErrorString(hipGetLastError()));
             s->st.cert_attempts++;
             s->st.cert_passes += cr.passes;
@@ -450,6 +453,7 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
                 s->cert_gap = s->cert_every;
             } else {
                 s->cert_gap = std::min(s->cert_gap * 2, 1 << 16);
+                if (cr.cap_hit) s->cert_cap = std::min(s->cert_cap * 2, s->cert_max_passes);
             }
             s->next_cert = s->next_round - 1 + s->cert_gap;
             pending = -1;

@@ -18,6 +18,7 @@
 #include "../../include/egs.h"
 #include "egs_games.h"
 #include "egs_internal.h"
+#include "egs_pack.h"
 
 namespace egs {
 
@@ -141,13 +142,13 @@ int egs_batch_add(egs_batch *b, int32_t n, int32_t n0, int64_t m, int32_t mg, co
                   const int32_t *succ, const int32_t *w) {
     if (!b || n < 1 || n0 < 0 || n0 > n || m < 1 || mg < 0 || !ptr || !succ || !w)
         return bfail(EGS_E_ARG, "egs_batch_add: bad game");
-    if (ptr[0] != 0 || ptr[n] != m) return bfail(EGS_E_ARG, "egs_batch_add: CSR offsets must run 0..m");
-    for (int v = 0; v < n; v++)
-        if (ptr[v + 1] - ptr[v] < 1) return bfail(EGS_E_ARG, "egs_batch_add: vertex %d has no out-edge", v);
-    int wm = 0;
-    for (int64_t e = 0; e < m; e++) {
-        if (succ[e] < 0 || succ[e] >= n) return bfail(EGS_E_ARG, "egs_batch_add: successor out of range");
-        wm = std::max(wm, w[e] < 0 ? -w[e] : w[e]);
+    int32_t wm = 0, dm = 0;
+    int64_t bad = -1;
+    switch (egs_pk_check(n, m, ptr, succ, w, &wm, &dm, &bad)) {
+    case EGS_PK_BAD_OFFSETS: return bfail(EGS_E_ARG, "egs_batch_add: CSR offsets must run 0..m");
+    case EGS_PK_NO_EDGE: return bfail(EGS_E_ARG, "egs_batch_add: vertex %lld has no out-edge", (long long)bad);
+    case EGS_PK_BAD_SUCC: return bfail(EGS_E_ARG, "egs_batch_add: successor out of range at edge %lld", (long long)bad);
+    default: break;
     }
     Staged g{n, n0, mg, wm, m, b->h_ptr.size(), b->h_succ.size()};
     b->games.push_back(g);
@@ -213,12 +214,12 @@ int egs_batch_upload(egs_batch *b) {
         const int32_t *su = b->h_succ.data() + g.succ_at, *wv = b->h_w.data() + g.succ_at;
         if (shift[i]) {
             packed.resize(g.m);
-            for (int64_t e = 0; e < g.m; e++) packed[e] = ((uint32_t)wv[e] << shift[i]) | (uint32_t)su[e];
+            egs_pk_pack32(g.m, su, wv, shift[i], packed.data());
             BHIP(hipMemcpy((void *)d.edges, packed.data(), 4 * (size_t)g.m, hipMemcpyHostToDevice));
             b->packed_games++;
         } else {
             wide.resize(g.m);
-            for (int64_t e = 0; e < g.m; e++) wide[e] = make_int2(su[e], wv[e]);
+            egs_pk_pack64(g.m, su, wv, (int32_t *)wide.data());
             BHIP(hipMemcpy((void *)d.edges, wide.data(), 8 * (size_t)g.m, hipMemcpyHostToDevice));
         }
     }
@@ -231,12 +232,12 @@ int egs_batch_solve(egs_batch *b, const egs_batch_options *o, egs_batch_stats *o
     egs_batch_options def{};
     if (!o) o = &def;
     BHIP(hipSetDevice(b->device));
-    const int threads = o->threads > 0 ? o->threads : 512;
+    const int threads = o->threads > 0 ? o->threads : 1024;
     if (threads != 256 && threads != 512 && threads != 1024)
         return bfail(EGS_E_ARG, "threads must be 256, 512 or 1024");
-    int lds_ints = o->lds_ints > 0 ? o->lds_ints : 16384;
+    int lds_ints = o->lds_ints > 0 ? o->lds_ints : 1 << 30;   // default: all the LDS a workgroup may hold
     lds_ints = std::min(lds_ints, (b->lds_limit - 2048) / 4);
-    const int per_cu = o->blocks_per_cu > 0 ? o->blocks_per_cu : 2;
+    const int per_cu = o->blocks_per_cu > 0 ? o->blocks_per_cu : 1;
     const int grid = std::max(1, b->cus * per_cu);
     const int ng = (int)b->games.size();
     std::vector<int> order(ng);
@@ -252,13 +253,14 @@ int egs_batch_solve(egs_batch *b, const egs_batch_options *o, egs_batch_stats *o
     BHIP(hipMemsetAsync(b->d_states, 0, sizeof(egs::GameState) * ng, b->stream));
     BHIP(hipMemcpyAsync(b->d_order, order.data(), 4 * (size_t)ng, hipMemcpyHostToDevice, b->stream));
     BHIP(hipMemsetAsync(b->d_counter, 0, 4, b->stream));
-    const int64_t budget = o->max_sweeps > 0 ? o->max_sweeps : INT64_MAX;
+    const int64_t max_sweeps = o->max_sweeps > 0 ? o->max_sweeps : INT64_MAX;
     egs::CertArgs ca{o->certify_every == 0 ? 16 : (o->certify_every < 0 ? 0 : o->certify_every),
-                     o->cert_max_passes > 0 ? o->cert_max_passes : 1024};
+                     o->cert_max_passes > 0 ? o->cert_max_passes : 1024, egs::kGameFirstCap,
+                     egs::kGameCandidates};
     BHIP(hipEventRecord(b->ev_a, b->stream));
     hipLaunchKernelGGL(egs::k_games, dim3(grid), dim3(threads), sizeof(int) * (size_t)lds_ints, b->stream,
                        (const egs::GameDesc *)b->d_descs, (const int *)b->d_order, ng, b->d_counter, lds_ints,
-                       budget, ca);
+                       (int64_t)INT64_MAX, max_sweeps, ca);
     BHIP(hipGetLastError());
     BHIP(hipEventRecord(b->ev_b, b->stream));
     BHIP(hipEventSynchronize(b->ev_b));

@@ -2,9 +2,13 @@
 // the serving loop (egs_serve.hip).
 //
 // A launch runs over a list of game descriptors.  A persistent workgroup claims one whole
-// game at a time from a device counter and runs in-place sweeps over its vertices until a
-// sweep changes nothing (a block-wide vote) or the launch's sweep budget for that game is
-// spent; the game's progress lives in its GameState, so a later launch resumes it.
+// game at a time from a device counter and runs in-place (Gauss-Seidel) sweeps over its
+// vertices until a complete sweep changes nothing (a block-wide vote), the launch's sweep
+// cap for that game is reached, or the workgroup's edge budget for the launch is spent.
+// A sweep may stop part-way at a chunk boundary: the game's GameState keeps the vertex
+// cursor and whether the sweep has changed anything so far, so a later launch resumes it.
+// A sweep that changed nothing proves the fixpoint even when it spanned launches: no value
+// moved while it ran, so every vertex was checked against the final values.
 // Values sit in LDS when the game fits the workgroup's LDS share, else in global memory.
 // Every `every` sweeps (doubling while proofs fail) the workgroup tries the certified TOP
 // proof of egs_accel.hip on its game.
@@ -24,6 +28,10 @@ struct GameState {
     int32_t converged;     // 1 once a sweep changed nothing
     int32_t certificates;
     int32_t started;       // values initialised
+    int32_t bf_cap;        // longest-path passes per candidate threshold (doubles while attempts run out)
+    int32_t cursor;        // first vertex of the current sweep not yet visited
+    int32_t dirty;         // the current sweep has changed a value
+    int32_t reserved;
 };
 
 struct GameDesc {
@@ -38,9 +46,14 @@ struct GameDesc {
     int shift, group;
 };
 
+constexpr int kGameFirstCap = 32;      // longest-path passes per candidate at a game's first attempt
+constexpr int kGameCandidates = 8;     // thresholds tried per attempt
+
 struct CertArgs {
     int every;             // sweeps between proof attempts; <= 0 = no proofs
-    int max_passes;        // longest-path passes per attempt
+    int max_passes;        // ceiling of the longest-path passes per candidate threshold
+    int first_cap;         // passes per candidate at a game's first attempt
+    int candidates;        // thresholds tried per attempt
 };
 
 __device__ __forceinline__ void game_edge(const GameDesc &b, int64_t k, int &succ, int &w) {
@@ -55,14 +68,19 @@ __device__ __forceinline__ void game_edge(const GameDesc &b, int64_t k, int &suc
     }
 }
 
-// One sweep by groups of G lanes; block-uniform `base` loop keeps waves whole for DPP.
+// Part of a sweep by groups of G lanes, from vertex `cursor`, chunk by chunk (a chunk is
+// one vertex per group) while `spent` < `budget`; returns the next cursor.  `spent` adds
+// the chunk's edge count, read from the offsets by every lane, so it stays block-uniform
+// and the loop keeps waves whole for the DPP reductions.
 template <int G>
-__device__ __forceinline__ bool game_sweep(const GameDesc &b, int *val, int64_t &edges_done, int &lifted) {
+__device__ __forceinline__ int game_sweep_part(const GameDesc &b, int *val, int cursor, int64_t budget,
+                                               int64_t &spent, int64_t &edges_done, int &lifted, bool &changed) {
     const int gl = threadIdx.x % G;
     const int gid = threadIdx.x / G;
     const int ngroups = blockDim.x / G;
-    bool changed = false;
-    for (int base = 0; base < b.n; base += ngroups) {
+    int base = cursor;
+    while (base < b.n && spent < budget) {
+        spent += b.ptr[min(base + ngroups, b.n)] - b.ptr[base];
         const int v = base + gid;
         const bool valid = v < b.n;
         int s = 0, e = 0, old = 0;
@@ -88,16 +106,22 @@ __device__ __forceinline__ bool game_sweep(const GameDesc &b, int *val, int64_t 
                 lifted++;
             }
         }
+        base += ngroups;
     }
-    return changed;
+    return min(base, b.n);
 }
 
 __device__ __forceinline__ bool game_in_d(int x, int B) { return x > B && x != EGS_TOP; }
 
-// The certified TOP proof inside the workgroup; returns the vertices set to TOP.
-__device__ int game_certify(const GameDesc &b, int *val, int max_passes, int64_t &passes_done) {
+// The certified TOP proof inside the workgroup (the steps of egs_accel.hip): every
+// admissible gap threshold, smallest first, gets the closure check and at most `cap`
+// longest-path passes; the first D that passes both is set to TOP.  Returns the vertices
+// set to TOP; `cap_hit` tells whether some candidate ran out of passes.
+__device__ int game_certify(const GameDesc &b, int *val, int cap, int max_cands, int64_t &passes_done,
+                            bool &cap_hit) {
     __shared__ int win[64], flo[64];
-    __shared__ int s_B, s_marked;
+    __shared__ unsigned long long s_cand;
+    __shared__ int s_marked;
     const int wmax = b.wmax;
     if (wmax <= 0) return 0;
     for (int i = threadIdx.x; i < 64; i += blockDim.x) { win[i] = 0; flo[i] = 0; }
@@ -111,120 +135,128 @@ __device__ int game_certify(const GameDesc &b, int *val, int max_passes, int64_t
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        s_B = -1;
-        int64_t above[40];       // above[j] = #{non-TOP values > W * 2^j}
-        int64_t acc = 0;
-        for (int k = 63; k >= 40; k--) acc += flo[k];
-        for (int j = 39; j >= 0; j--) { acc += flo[j]; above[j] = acc; }
-        for (int j = 1; j < 40; j++) {
-            const int64_t B = (int64_t)wmax << j;
-            if (B > (int64_t)b.mg - wmax) break;
-            if (win[j] == 0 && above[j] > 0) { s_B = (int)B; break; }
+        // bit j set: B_j = W * 2^j has an empty window, non-TOP values above it, B_j <= MG - W
+        unsigned long long mask = 0;
+        int64_t above = 0;                    // #{non-TOP values > W * 2^j} while j descends
+        for (int k = 63; k >= 40; k--) above += flo[k];
+        for (int j = 39; j >= 1; j--) {
+            above += flo[j];
+            if (((int64_t)wmax << j) <= (int64_t)b.mg - wmax && win[j] == 0 && above > 0) mask |= 1ull << j;
         }
+        s_cand = mask;
         s_marked = 0;
     }
     __syncthreads();
-    const int B = s_B;
-    if (B < 0) return 0;
-    bool bad = false;
-    for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
-        b.h[v] = 0;
-        if (!game_in_d(val[v], B)) continue;
-        const int s = b.ptr[v], e = b.ptr[v + 1];
-        if (v >= b.n0) {
-            int best = -1, arg = s;
-            for (int k = s; k < e; k++) {
-                int u, w;
-                game_edge(b, k, u, w);
-                const int c = egs_ominus(val[u], w, b.mg);
-                if (c > best) { best = c; arg = k; }
-            }
-            b.sigma[v] = arg;
-            int u, w;
-            game_edge(b, arg, u, w);
-            if (val[u] <= B) bad = true;
-        } else {
-            for (int k = s; k < e; k++) {
-                int u, w;
-                game_edge(b, k, u, w);
-                if (val[u] <= B) bad = true;
-            }
-        }
-    }
-    if (__syncthreads_or(bad)) return 0;
+    unsigned long long cand = s_cand;
     const long long scale = (long long)b.n + 1;
     const long long bound = (long long)b.n * (scale * wmax + 1);
-    bool converged = false;
-    for (int p = 0; p < max_passes; p++) {
-        bool ch = false, over = false;
+    for (int tried = 0; cand && tried < max_cands; tried++) {
+        const int j = __ffsll(cand) - 1;
+        cand &= cand - 1;
+        const int B = (int)((int64_t)wmax << j);
+        bool bad = false;
         for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
+            b.h[v] = 0;
             if (!game_in_d(val[v], B)) continue;
-            long long best = 0;
-            const int s = (v >= b.n0) ? b.sigma[v] : b.ptr[v];
-            const int e = (v >= b.n0) ? b.sigma[v] + 1 : b.ptr[v + 1];
-            for (int k = s; k < e; k++) {
+            const int s = b.ptr[v], e = b.ptr[v + 1];
+            if (v >= b.n0) {
+                int best = -1, arg = s;
+                for (int k = s; k < e; k++) {
+                    int u, w;
+                    game_edge(b, k, u, w);
+                    const int c = egs_ominus(val[u], w, b.mg);
+                    if (c > best) { best = c; arg = k; }
+                }
+                b.sigma[v] = arg;
                 int u, w;
-                game_edge(b, k, u, w);
-                if (game_in_d(val[u], B)) {
-                    const long long c = scale * w + 1 + b.h[u];
-                    if (c > best) best = c;
+                game_edge(b, arg, u, w);
+                if (val[u] <= B) bad = true;
+            } else {
+                for (int k = s; k < e; k++) {
+                    int u, w;
+                    game_edge(b, k, u, w);
+                    if (val[u] <= B) bad = true;
                 }
             }
-            if (best > b.h[v]) {
-                b.h[v] = best;
-                ch = true;
-                over |= best > bound;
-            }
         }
-        passes_done++;
-        const int raised = __syncthreads_count(ch);
-        if (__syncthreads_or(over)) break;
-        if (raised == 0) { converged = true; break; }
+        if (__syncthreads_or(bad)) continue;
+        bool converged = false, diverged = false;
+        for (int p = 0; p < cap; p++) {
+            bool ch = false, over = false;
+            for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
+                if (!game_in_d(val[v], B)) continue;
+                long long best = 0;
+                const int s = (v >= b.n0) ? b.sigma[v] : b.ptr[v];
+                const int e = (v >= b.n0) ? b.sigma[v] + 1 : b.ptr[v + 1];
+                for (int k = s; k < e; k++) {
+                    int u, w;
+                    game_edge(b, k, u, w);
+                    if (game_in_d(val[u], B)) {
+                        const long long c = scale * w + 1 + b.h[u];
+                        if (c > best) best = c;
+                    }
+                }
+                if (best > b.h[v]) {
+                    b.h[v] = best;
+                    ch = true;
+                    over |= best > bound;
+                }
+            }
+            passes_done++;
+            const int raised = __syncthreads_count(ch);
+            if (__syncthreads_or(over)) { diverged = true; break; }
+            if (raised == 0) { converged = true; break; }
+        }
+        if (!converged) {
+            if (!diverged) cap_hit = true;
+            continue;
+        }
+        int mine = 0;
+        for (int v = threadIdx.x; v < b.n; v += blockDim.x)
+            if (game_in_d(val[v], B)) { val[v] = EGS_TOP; mine++; }
+        if (mine) atomicAdd(&s_marked, mine);
+        __syncthreads();
+        return s_marked;
     }
-    if (!converged) return 0;
-    int mine = 0;
-    for (int v = threadIdx.x; v < b.n; v += blockDim.x)
-        if (game_in_d(val[v], B)) { val[v] = EGS_TOP; mine++; }
-    if (mine) atomicAdd(&s_marked, mine);
-    __syncthreads();
-    return s_marked;
+    return 0;
 }
 
-// Advance one game by at most `budget` sweeps (the state carries it across launches).
+// Advance one game until its fixpoint, a due proof attempt, `max_sweeps` more complete
+// sweeps, or `budget` edges (checked per chunk).  `sst` is the workgroup's shared copy of
+// the game's state; the proof itself runs in game_prove, outside the lane-width variants,
+// so the kernel carries one copy of it.  Adds the edges charged to the budget to `spent`
+// and the complete sweeps to `sweeps` (both block-uniform); returns whether it stopped
+// because a proof attempt is due.
 template <int G>
-__device__ void game_run(const GameDesc &b, int *val, int64_t budget, const CertArgs &ca) {
-    __shared__ GameState s_st;
-    if (threadIdx.x == 0) s_st = *b.state;
+__device__ bool game_run(const GameDesc &b, int *val, GameState &sst, int64_t budget, int64_t max_sweeps,
+                         const CertArgs &ca, int64_t &spent, int64_t &sweeps) {
+    // the fields the loop changes, as scalars (a whole-struct copy ends up in scratch)
+    int cursor = sst.cursor;
+    bool converged = sst.converged != 0, dirty = sst.dirty != 0;
+    const int64_t sweeps0 = sst.sweeps, next_cert = sst.next_cert;
     __syncthreads();
-    GameState st = s_st;
-    if (!st.started) {
-        for (int v = threadIdx.x; v < b.n; v += blockDim.x) val[v] = 0;
-        st.started = 1;
-        st.gap = ca.every;
-        st.next_cert = ca.every > 0 ? ca.every : INT64_MAX;
-    }
-    __syncthreads();
-    int64_t edges_done = 0, passes = 0;
+    int64_t edges_done = 0, used = 0, done = 0;
     int lifted = 0;
-    int64_t done_here = 0;
-    bool go = !st.converged;
-    while (go && done_here < budget) {
-        const bool ch = game_sweep<G>(b, val, edges_done, lifted);
-        go = __syncthreads_or(ch) != 0;
-        st.sweeps++;
-        done_here++;
-        if (go && ca.every > 0 && st.sweeps >= st.next_cert) {
-            const int marked = game_certify(b, val, ca.max_passes, passes);
-            if (marked > 0) {
-                st.certificates++;
-                st.gap = ca.every;
-            } else {
-                st.gap = min(st.gap * 2, 1 << 16);
-            }
-            st.next_cert = st.sweeps + st.gap;
-            __syncthreads();
+    bool proof_due = false;
+    while (!converged && used < budget && done < max_sweeps) {
+        bool ch = false;
+        cursor = game_sweep_part<G>(b, val, cursor, budget, used, edges_done, lifted, ch);
+        dirty = __syncthreads_or(ch || dirty) != 0;
+        if (cursor < b.n) break;                    // budget spent part-way through the sweep
+        done++;
+        cursor = 0;
+        if (!dirty) {
+            converged = true;
+            break;
+        }
+        dirty = false;
+        if (ca.every > 0 && sweeps0 + done >= next_cert) {
+            proof_due = true;
+            break;
         }
     }
+    spent += used;
+    sweeps += done;
     __shared__ unsigned long long red[2];
     if (threadIdx.x == 0) { red[0] = 0; red[1] = 0; }
     __syncthreads();
@@ -232,21 +264,51 @@ __device__ void game_run(const GameDesc &b, int *val, int64_t budget, const Cert
     if (lifted) atomicAdd(&red[1], (unsigned long long)lifted);
     __syncthreads();
     if (threadIdx.x == 0) {
-        st.edges += (int64_t)red[0];
-        st.lifted += (int64_t)red[1];
-        st.cert_passes += passes;
-        st.converged = go ? 0 : 1;
-        *b.state = st;
+        sst.edges += (int64_t)red[0];
+        sst.lifted += (int64_t)red[1];
+        sst.sweeps = sweeps0 + done;
+        sst.cursor = cursor;
+        sst.converged = converged ? 1 : 0;
+        sst.dirty = dirty ? 1 : 0;
     }
+    __syncthreads();
+    return proof_due;
+}
+
+// One proof attempt at a sweep boundary; schedules the next one (spacing back to `every`
+// after a proof, doubled after a failure, pass cap doubled when a candidate ran out).
+__device__ void game_prove(const GameDesc &b, int *val, GameState &sst, const CertArgs &ca) {
+    const int cap = sst.bf_cap;
+    __syncthreads();
+    int64_t passes = 0;
+    bool cap_hit = false;
+    const int marked = game_certify(b, val, cap, ca.candidates, passes, cap_hit);
+    const bool hit = __syncthreads_or(cap_hit) != 0;
+    if (threadIdx.x == 0) {
+        sst.cert_passes += passes;
+        if (marked > 0) {
+            sst.certificates++;
+            sst.gap = ca.every;
+        } else {
+            sst.gap = min(sst.gap * 2, 1 << 16);
+            if (hit) sst.bf_cap = min(sst.bf_cap * 2, ca.max_passes);
+        }
+        sst.next_cert = sst.sweeps + sst.gap;
+    }
+    __syncthreads();
 }
 
 // The persistent many-games kernel (internal linkage: each including unit keeps a copy).
+// Each workgroup claims games in `order` until they run out or its `wg_budget` edges for
+// this launch are spent; a game gets at most `max_sweeps` complete sweeps per launch.
 static __global__ __launch_bounds__(1024) void k_games(const GameDesc *__restrict__ games, const int *__restrict__ order,
-                                                int ngames, int *counter, int lds_cap, int64_t budget,
-                                                CertArgs ca) {
+                                                int ngames, int *counter, int lds_cap, int64_t wg_budget,
+                                                int64_t max_sweeps, CertArgs ca) {
     extern __shared__ int lds_val[];
     __shared__ int s_idx;
-    for (;;) {
+    __shared__ GameState s_st;
+    int64_t left = wg_budget;
+    while (left > 0) {
         if (threadIdx.x == 0) s_idx = atomicAdd(counter, 1);
         __syncthreads();
         const int idx = s_idx;
@@ -255,23 +317,49 @@ static __global__ __launch_bounds__(1024) void k_games(const GameDesc *__restric
         const GameDesc b = games[order[idx]];
         const bool in_lds = b.n <= lds_cap;
         int *val = in_lds ? lds_val : b.values;
-        if (in_lds) {
-            // resumed games bring their values in; new ones are zeroed by game_run
-            for (int v = threadIdx.x; v < b.n; v += blockDim.x) lds_val[v] = b.values[v];
-            __syncthreads();
+        if (threadIdx.x == 0) {
+            s_st = *b.state;
+            if (!s_st.started) {
+                s_st.started = 1;
+                s_st.gap = ca.every;
+                s_st.next_cert = ca.every > 0 ? ca.every : INT64_MAX;
+                s_st.bf_cap = min(ca.first_cap, ca.max_passes);
+                s_st.cursor = 0;
+                s_st.dirty = 0;
+                s_idx = -1;                // marks a fresh game for the value set-up below
+            }
         }
-        switch (b.group) {
-        case 1: game_run<1>(b, val, budget, ca); break;
-        case 2: game_run<2>(b, val, budget, ca); break;
-        case 4: game_run<4>(b, val, budget, ca); break;
-        case 8: game_run<8>(b, val, budget, ca); break;
-        case 16: game_run<16>(b, val, budget, ca); break;
-        case 32: game_run<32>(b, val, budget, ca); break;
-        default: game_run<64>(b, val, budget, ca); break;
+        __syncthreads();
+        const bool fresh = s_idx < 0;
+        if (fresh) {
+            for (int v = threadIdx.x; v < b.n; v += blockDim.x) val[v] = 0;
+        } else if (in_lds) {
+            for (int v = threadIdx.x; v < b.n; v += blockDim.x) lds_val[v] = b.values[v];
+        }
+        __syncthreads();
+        int64_t sweeps_left = max_sweeps;
+        for (;;) {
+            int64_t spent = 0, sweeps = 0;
+            bool due;
+            switch (b.group) {
+            case 1: due = game_run<1>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            case 2: due = game_run<2>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            case 4: due = game_run<4>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            case 8: due = game_run<8>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            case 16: due = game_run<16>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            case 32: due = game_run<32>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            default: due = game_run<64>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            }
+            left -= spent;
+            sweeps_left -= sweeps;
+            if (!due) break;
+            game_prove(b, val, s_st, ca);
+            if (left <= 0 || sweeps_left <= 0) break;
         }
         if (in_lds) {
             for (int v = threadIdx.x; v < b.n; v += blockDim.x) b.values[v] = lds_val[v];
         }
+        if (threadIdx.x == 0) *b.state = s_st;
         __syncthreads();
     }
 }

@@ -70,9 +70,16 @@ struct CertBuffers {
     int *pinned;      // host pinned: 2 x 64 + 16 ints
 };
 struct CertResult {
-    int threshold, passes, marked, closure_failed;
+    int threshold;       // last threshold tried
+    int tried;           // candidates tried
+    int passes;          // longest-path passes over all candidates
+    int marked;          // vertices set to TOP
+    int closure_failed;  // candidates rejected by the closure check
+    int cap_hit;         // some candidate ran out of passes (neither converged nor diverged)
 };
 hipError_t launch_abs_wmax(const DevGame &g, int *out, hipStream_t st);
-int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int max_passes, hipStream_t st, CertResult &res);
+int gap_candidates(const int *win, const int *flo, int wmax, int mg, int64_t *out, int max_out);
+int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int cap, int max_cands, hipStream_t st,
+                CertResult &res);
 
 }  // namespace egs

@@ -20,6 +20,7 @@
 
 #include "../../include/egs.h"
 #include "egs_games.h"
+#include "egs_pack.h"
 
 namespace egs {
 int pack_shift(int n, int wmax);
@@ -79,7 +80,10 @@ struct Slot {
     size_t at = 0, bytes = 0;
     int n = 0;
     int64_t m = 0;
+    int64_t seq = 0;      // admission order
 };
+
+constexpr int kSliceKedges = 2048;   // default edge budget per workgroup per slice, in thousands
 
 size_t footprint(int n, int64_t m, int wmax) {
     const int shift = egs::pack_shift(n, wmax);
@@ -109,6 +113,8 @@ struct egs_server {
     int lds_limit = 0, cus = 0;
     bool slice_running = false;
     double last_slice_ms = 0;
+    int64_t admitted = 0;     // admissions so far (slot sequence numbers)
+    int hi_slot = -1;         // highest slot index ever used (descriptor/state copies cover 0..hi_slot)
 };
 
 extern "C" {
@@ -199,13 +205,14 @@ int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t m
                       const int32_t *succ, const int32_t *w) {
     if (!s || n < 1 || n0 < 0 || n0 > n || m < 1 || mg < 0 || !ptr || !succ || !w)
         return sfail(EGS_E_ARG, "egs_server_submit: bad game");
-    if (ptr[0] != 0 || ptr[n] != m) return sfail(EGS_E_ARG, "egs_server_submit: CSR offsets must run 0..m");
-    int wm = 0;
-    for (int v = 0; v < n; v++)
-        if (ptr[v + 1] - ptr[v] < 1) return sfail(EGS_E_ARG, "egs_server_submit: vertex %d has no out-edge", v);
-    for (int64_t e = 0; e < m; e++) {
-        if (succ[e] < 0 || succ[e] >= n) return sfail(EGS_E_ARG, "egs_server_submit: successor out of range");
-        wm = std::max(wm, w[e] < 0 ? -w[e] : w[e]);
+    int32_t wm = 0, dm = 0;
+    int64_t bad = -1;
+    switch (egs_pk_check(n, m, ptr, succ, w, &wm, &dm, &bad)) {
+    case EGS_PK_BAD_OFFSETS: return sfail(EGS_E_ARG, "egs_server_submit: CSR offsets must run 0..m");
+    case EGS_PK_NO_EDGE: return sfail(EGS_E_ARG, "egs_server_submit: vertex %lld has no out-edge", (long long)bad);
+    case EGS_PK_BAD_SUCC:
+        return sfail(EGS_E_ARG, "egs_server_submit: successor out of range at edge %lld", (long long)bad);
+    default: break;
     }
     int ticket = -1;
     for (int i = 0; i < s->max_games; i++)
@@ -232,13 +239,10 @@ int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t m
     SHIP(hipStreamSynchronize(s->copy));
     char *stg = (char *)s->staging;
     memcpy(stg, ptr, ptr_b);
-    if (shift) {
-        uint32_t *pe = (uint32_t *)(stg + ptr_b);
-        for (int64_t e = 0; e < m; e++) pe[e] = ((uint32_t)w[e] << shift) | (uint32_t)succ[e];
-    } else {
-        int2 *pe = (int2 *)(stg + ptr_b);
-        for (int64_t e = 0; e < m; e++) pe[e] = make_int2(succ[e], w[e]);
-    }
+    if (shift)
+        egs_pk_pack32(m, succ, w, shift, (uint32_t *)(stg + ptr_b));
+    else
+        egs_pk_pack64(m, succ, w, (int32_t *)(stg + ptr_b));
     char *base = (char *)s->arena + at;
     egs::GameDesc d{};
     d.ptr = (const int *)base;
@@ -266,6 +270,8 @@ int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t m
     sl.bytes = bytes;
     sl.n = n;
     sl.m = m;
+    sl.seq = s->admitted++;
+    s->hi_slot = std::max(s->hi_slot, ticket);
     s->descs[ticket] = d;
     memset(&s->states[ticket], 0, sizeof(egs::GameState));
     return ticket;
@@ -292,33 +298,35 @@ int egs_server_launch(egs_server *s, const egs_serve_options *o) {
         }
     }
     if (s->active.empty()) return 0;
-    // longest remaining work first: size times edges, less what was already swept
-    std::stable_sort(s->active.begin(), s->active.end(), [&](int a, int c) {
-        return (double)s->slots[a].n * s->slots[a].m > (double)s->slots[c].n * s->slots[c].m;
-    });
-    const int threads = o->threads > 0 ? o->threads : 512;
+    // oldest admission first: every workgroup's edge budget bounds the slice, so the games
+    // at the front of the order progress every slice and none waits behind a large one
+    std::stable_sort(s->active.begin(), s->active.end(),
+                     [&](int a, int c) { return s->slots[a].seq < s->slots[c].seq; });
+    const int threads = o->threads > 0 ? o->threads : 1024;
     if (threads != 256 && threads != 512 && threads != 1024) return sfail(EGS_E_ARG, "threads must be 256, 512 or 1024");
-    int lds_ints = o->lds_ints > 0 ? o->lds_ints : 16384;
+    int lds_ints = o->lds_ints > 0 ? o->lds_ints : 1 << 30;   // default: all the LDS a workgroup may hold
     lds_ints = std::min(lds_ints, (s->lds_limit - 2048) / 4);
-    const int grid = std::max(1, s->cus * (o->blocks_per_cu > 0 ? o->blocks_per_cu : 2));
-    const int64_t budget = o->slice_sweeps > 0 ? o->slice_sweeps : 64;
+    const int grid = std::max(1, s->cus * (o->blocks_per_cu > 0 ? o->blocks_per_cu : 1));
+    const int64_t max_sweeps = o->slice_sweeps > 0 ? o->slice_sweeps : INT64_MAX;
+    const int64_t wg_budget = 1000 * (int64_t)(o->slice_kedges > 0 ? o->slice_kedges : kSliceKedges);
     egs::CertArgs ca{o->certify_every == 0 ? 16 : (o->certify_every < 0 ? 0 : o->certify_every),
-                     o->cert_max_passes > 0 ? o->cert_max_passes : 1024};
+                     o->cert_max_passes > 0 ? o->cert_max_passes : 1024, egs::kGameFirstCap,
+                     egs::kGameCandidates};
     // descriptors of the slice's games, then the uploads, then the kernel
     // (a previous slice's copies have completed: wait() synchronised on its event)
     std::copy(s->active.begin(), s->active.end(), s->order);
-    SHIP(hipMemcpyAsync(s->d_descs, s->descs, sizeof(egs::GameDesc) * s->max_games, hipMemcpyHostToDevice,
-                        s->compute));
+    SHIP(hipMemcpyAsync(s->d_descs, s->descs, sizeof(egs::GameDesc) * (size_t)(s->hi_slot + 1),
+                        hipMemcpyHostToDevice, s->compute));
     SHIP(hipMemcpyAsync(s->d_order, s->order, 4 * s->active.size(), hipMemcpyHostToDevice, s->compute));
     SHIP(hipMemsetAsync(s->d_counter, 0, 4, s->compute));
     SHIP(hipStreamWaitEvent(s->compute, s->ev_upload, 0));
     SHIP(hipEventRecord(s->ev_a, s->compute));
     hipLaunchKernelGGL(egs::k_games, dim3(grid), dim3(threads), sizeof(int) * (size_t)lds_ints, s->compute,
                        (const egs::GameDesc *)s->d_descs, (const int *)s->d_order, (int)s->active.size(),
-                       s->d_counter, lds_ints, budget, ca);
+                       s->d_counter, lds_ints, wg_budget, max_sweeps, ca);
     SHIP(hipGetLastError());
-    SHIP(hipMemcpyAsync(s->states, s->d_states, sizeof(egs::GameState) * s->max_games, hipMemcpyDeviceToHost,
-                        s->compute));
+    SHIP(hipMemcpyAsync(s->states, s->d_states, sizeof(egs::GameState) * (size_t)(s->hi_slot + 1),
+                        hipMemcpyDeviceToHost, s->compute));
     SHIP(hipEventRecord(s->ev_slice, s->compute));
     s->slice_running = true;
     return (int)s->active.size();

@@ -32,7 +32,7 @@ class ServeOptions(ctypes.Structure):
         ("slice_sweeps", ctypes.c_int32),
         ("certify_every", ctypes.c_int32),
         ("cert_max_passes", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("slice_kedges", ctypes.c_int32),
     ]
 
 
@@ -64,9 +64,9 @@ def _check(rc, what):
 
 
 def serve_options(threads=0, blocks_per_cu=0, lds_ints=0, lanes_per_edge_x16=0, group_min=0, group_max=0,
-                  slice_sweeps=0, certify_every=0, cert_max_passes=0):
+                  slice_sweeps=0, certify_every=0, cert_max_passes=0, slice_kedges=0):
     return ServeOptions(threads, blocks_per_cu, lds_ints, lanes_per_edge_x16, group_min, group_max, slice_sweeps,
-                        certify_every, cert_max_passes, 0)
+                        certify_every, cert_max_passes, slice_kedges)
 
 
 class Server:

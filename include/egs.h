@@ -152,16 +152,16 @@ int egs_solver_count_violations(egs_solver *s, int64_t *violations);
 typedef struct egs_batch egs_batch;
 
 typedef struct egs_batch_options {
-    int32_t threads;            /* workgroup size: 256, 512 or 1024; 0 = 512 */
-    int32_t blocks_per_cu;      /* persistent workgroups per compute unit; 0 = 2 */
+    int32_t threads;            /* workgroup size: 256, 512 or 1024; 0 = 1024 */
+    int32_t blocks_per_cu;      /* persistent workgroups per compute unit; 0 = 1 */
     int32_t lds_ints;           /* LDS values per workgroup: games up to this many vertices
-                                   keep their values in LDS; 0 = 16384 */
+                                   keep their values in LDS; 0 = all a workgroup may hold */
     int32_t lanes_per_edge_x16; /* lanes per vertex = smallest power of two >= average
                                    degree * this / 16; 0 = 8 */
     int32_t group_min, group_max; /* clamp of the lanes per vertex; 0 = 1 / 64 */
     int64_t max_sweeps;         /* per-game sweep cap; <= 0 = none */
     int32_t certify_every;      /* sweeps between TOP proofs per game; 0 = 16, < 0 = never */
-    int32_t cert_max_passes;    /* longest-path passes per proof; 0 = 1024 */
+    int32_t cert_max_passes;    /* ceiling of the longest-path passes per proof candidate; 0 = 1024 */
 } egs_batch_options;
 
 typedef struct egs_batch_stats {
@@ -203,15 +203,17 @@ const char *egs_batch_last_error(void);
 typedef struct egs_server egs_server;
 
 typedef struct egs_serve_options {
-    int32_t threads;            /* 256, 512 or 1024; 0 = 512 */
-    int32_t blocks_per_cu;      /* 0 = 2 */
-    int32_t lds_ints;           /* 0 = 16384 */
+    int32_t threads;            /* 256, 512 or 1024; 0 = 1024 */
+    int32_t blocks_per_cu;      /* 0 = 1 */
+    int32_t lds_ints;           /* 0 = all a workgroup may hold */
     int32_t lanes_per_edge_x16; /* 0 = 8 */
     int32_t group_min, group_max;
-    int32_t slice_sweeps;       /* sweeps per game per slice; 0 = 64 */
+    int32_t slice_sweeps;       /* cap of complete sweeps per game per slice; 0 = none */
     int32_t certify_every;      /* 0 = 16, < 0 = never */
     int32_t cert_max_passes;    /* 0 = 1024 */
-    int32_t reserved;
+    int32_t slice_kedges;       /* edge budget of each workgroup per slice, in thousands of
+                                   edges (bounds the slice time; sweeps resume part-way);
+                                   0 = 2048 */
 } egs_serve_options;
 
 int egs_server_create(int device, int64_t arena_bytes, int32_t max_games, egs_server **out);

@@ -123,3 +123,26 @@ def test_batch_with_and_without_proofs(oracle):
             vals = b.all_values()
             for i, ref in enumerate(refs):
                 assert np.array_equal(vals[i], ref), (every, i, np.count_nonzero(vals[i] != ref))
+
+
+def test_batch_proof_fallback_games(oracle):
+    """Batched-family games (seed 4, indices 1651, 737, 1380) on which the lowest gap
+    threshold's set holds finite vertices, so a proof attempt has to move on to higher
+    thresholds: exact, and the same alone and in the mix."""
+    import egsolver_amd as E
+    idxs = (1651, 737, 1380)
+    games = [E.generate(E.family_params(E.games.FAMILY_BATCH, 4, i), stream=i) for i in idxs]
+    _say(f"proof fallback games {[(g.n, g.m) for g in games]}")
+    refs = [lfp(oracle, to_oracle_game(g), threads=8) for g in games]
+    with E.Batch() as b:
+        for g in games:
+            b.add(g)
+        b.upload()
+        for every in (16, 8, 4):
+            st = b.solve(E.batch_options(certify_every=every))
+            assert st["converged"] == len(games)
+            for i, ref in enumerate(refs):
+                assert np.array_equal(b.values(i), ref), (every, idxs[i])
+    for g, ref in zip(games, refs):
+        alone, st = E.solve(g)
+        assert np.array_equal(alone, ref)

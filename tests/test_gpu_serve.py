@@ -65,3 +65,17 @@ def test_server_capacity_and_reuse():
         assert st["sweeps"] >= 1
         t1 = srv.submit(reqs[1].game)
         assert t1 is not None
+
+
+def test_served_partial_sweeps_exact(oracle):
+    """A per-workgroup budget of a few thousand edges stops sweeps part-way in nearly every
+    slice; the resumed sweeps still end on the exact least fixpoint."""
+    import egsolver_amd as E
+    reqs = _requests(7, 10, 1500, 0.0)
+    _say(f"serve partial sweeps, sizes {[r.game.n for r in reqs]}")
+    arena = sum(E.Server.footprint(r.game) for r in reqs) + (1 << 20)
+    done, wall, slices = E.run_served(reqs, arena, E.serve_options(slice_kedges=4), max_games=16,
+                                      keep_values=True)
+    assert slices > 10
+    for r in done:
+        assert np.array_equal(r.values, lfp(oracle, to_oracle_game(r.game))), (r.rid, r.game.n)

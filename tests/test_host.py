@@ -211,3 +211,48 @@ def test_cli_help_and_errors():
     assert r.returncode != 0 and "CONVERSION" in r.stderr
     r = _run("--input", "/nonexistent/file.mpg")
     assert r.returncode == 1 and "opening" in r.stderr
+
+
+def test_host_check_and_pack():
+    """The host-side CSR check and edge packing inside libegs.so (no device needed)."""
+    import ctypes
+    lib = N.egs()
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    chk = lib.egs_pk_check
+    chk.restype = ctypes.c_int
+    chk.argtypes = [ctypes.c_int32, ctypes.c_int64, i32p, i32p, i32p, i32p, i32p, ctypes.POINTER(ctypes.c_int64)]
+
+    def run(ptr, succ, w):
+        ptr, succ, w = (np.ascontiguousarray(a, np.int32) for a in (ptr, succ, w))
+        wm, dm, bad = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        rc = chk(len(ptr) - 1, len(succ), ptr.ctypes.data_as(i32p), succ.ctypes.data_as(i32p),
+                 w.ctypes.data_as(i32p), ctypes.byref(wm), ctypes.byref(dm), ctypes.byref(bad))
+        return rc, wm.value, dm.value, bad.value
+
+    g = E.generate(E.GenParams(n=50000, seed=9, deg_kind=1, deg_lo=1, deg_hi=9, wmax=777))
+    rc, wm, dm, _ = run(g.ptr, g.succ, g.w)
+    assert rc == 0 and wm == int(np.abs(g.w).max()) and dm == int(np.diff(g.ptr).max())
+    assert run([0, 1, 2], [1, 0, 1], [1, 2, 3])[0] == -1                 # ptr[n] != m
+    rc, _, _, bad = run([0, 2, 2, 3], [1, 0, 1], [1, 2, 3])             # vertex 1 has no out-edge
+    assert (rc, bad) == (-2, 1)
+    succ = g.succ.copy()
+    succ[40000] = g.n
+    rc, _, _, bad = run(g.ptr, succ, g.w)
+    assert (rc, bad) == (-3, 40000)
+    w = g.w.copy()
+    w[123] = -(2**31 - 1)
+    assert run(g.ptr, g.succ, w)[1] == 2**31 - 1
+
+    p32 = lib.egs_pk_pack32
+    p32.restype = None
+    p32.argtypes = [ctypes.c_int64, i32p, i32p, ctypes.c_int, ctypes.c_void_p]
+    out = np.empty(g.m, np.uint32)
+    p32(g.m, g.succ.ctypes.data_as(i32p), g.w.ctypes.data_as(i32p), 16, out.ctypes.data)
+    assert np.array_equal(out & 0xFFFF, g.succ.astype(np.uint32))
+    assert np.array_equal(out.view(np.int32) >> 16, g.w)
+    p64 = lib.egs_pk_pack64
+    p64.restype = None
+    p64.argtypes = [ctypes.c_int64, i32p, i32p, ctypes.c_void_p]
+    out = np.empty(2 * g.m, np.int32)
+    p64(g.m, g.succ.ctypes.data_as(i32p), g.w.ctypes.data_as(i32p), out.ctypes.data)
+    assert np.array_equal(out[0::2], g.succ) and np.array_equal(out[1::2], g.w)
