@@ -219,32 +219,16 @@ def batched(E, args):
 
 
 def serving(E, args):
+    """Continuous batching against one game per launch on the same Poisson stream."""
     from egsolver_amd.serve import Request, latency_stats, run_plain, run_served, serve_options
     R = args.serve_requests
     t0 = time.perf_counter()
     games = gen_many(E, E.games.FAMILY_SERVE, 5, R)
     log(f"serving: {R} requests generated in {time.perf_counter() - t0:.1f}s")
-    # capacity of the batched kernel on this distribution, to set the arrival rate
-    with E.Batch() as b:
-        for g in games[: min(R, 128)]:
-            b.add(g)
-        b.upload()
-        st = b.solve()
-        cap = min(R, 128) / (st["ms"] * 1e-3)
-    rate = args.serve_load * cap
-    rng = np.random.default_rng(5)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=R))
-    arrivals -= arrivals[0]
-    reqs = [Request(rid=i, arrival=float(arrivals[i]), game=games[i]) for i in range(R)]
     arena = int(args.serve_arena_gb * (1 << 30))
-    opt = serve_options(slice_sweeps=args.slice_sweeps)
-    done, wall, slices = run_served(reqs, arena, opt, max_games=4096)
-    served = latency_stats(done)
-    served.update({"solves_per_s": R / wall, "wall_s": wall, "slices": slices})
-    # the same stream, one game per launch (first requests only: the plain path saturates)
-    P = min(R, args.serve_plain_requests)
-    nmax = max(g.n for g in games[:P])
-    mmax = max(g.m for g in games[:P])
+    opt = serve_options(slice_kedges=args.slice_kedges)
+    nmax = max(g.n for g in games)
+    mmax = max(g.m for g in games)
     solver = E.Solver(nmax, mmax)
 
     def one(g):
@@ -252,17 +236,36 @@ def serving(E, args):
         solver.prepare()
         solver.iterate()
         return solver.values()
+    # closed-loop capacities (every request present at t = 0) on the first K requests
+    K = min(R, args.serve_capacity_requests)
+    done, wall, _ = run_served([Request(rid=i, arrival=0.0, game=games[i]) for i in range(K)], arena, opt)
+    cap = K / wall
+    P = min(R, args.serve_plain_requests)
+    _, pwall = run_plain([Request(rid=i, arrival=0.0, game=games[i]) for i in range(P)], one)
+    plain_cap = P / pwall
+    log(f"serving: closed-loop capacity served {cap:.1f}/s, plain {plain_cap:.1f}/s")
+    rate = args.serve_load * cap
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=R))
+    arrivals -= arrivals[0]
+    reqs = [Request(rid=i, arrival=float(arrivals[i]), game=games[i]) for i in range(R)]
+    done, wall, slices = run_served(reqs, arena, opt)
+    served = latency_stats(done)
+    served.update({"solves_per_s": R / wall, "wall_s": wall, "slices": slices, "closed_loop_solves_per_s": cap})
+    # the same stream, one game per launch (first P requests: at this rate the plain path
+    # falls behind and its queue grows)
     preqs = [Request(rid=i, arrival=float(arrivals[i]), game=games[i]) for i in range(P)]
     pdone, pwall = run_plain(preqs, one)
     solver.close()
     plain = latency_stats(pdone)
-    plain.update({"solves_per_s": P / pwall, "requests": P})
+    plain.update({"solves_per_s": P / pwall, "requests": P, "closed_loop_solves_per_s": plain_cap})
     log(f"serving: served p50 {served['p50_ms']:.1f} ms p99 {served['p99_ms']:.1f} ms {served['solves_per_s']:.1f}/s; "
         f"plain p50 {plain['p50_ms']:.1f} p99 {plain['p99_ms']:.1f} {plain['solves_per_s']:.1f}/s")
     return {"workload": f"{R} requests of the serving family (n log-uniform 1e3..5e5, otherwise as batched), "
-                        f"Poisson arrivals at {rate:.1f}/s = {args.serve_load:.2f} x measured batched capacity, seed 5",
+                        f"Poisson arrivals at {rate:.1f}/s = {args.serve_load:.2f} x the served closed-loop "
+                        f"capacity, seed 5",
             "served": served, "plain_one_game_per_launch": plain, "arrival_rate_per_s": rate,
-            "arena_gb": args.serve_arena_gb}
+            "capacity_speedup_vs_plain": cap / plain_cap, "arena_gb": args.serve_arena_gb}
 
 
 def main():
@@ -282,11 +285,12 @@ def main():
     ap.add_argument("--batch-games", type=int, default=4096)
     ap.add_argument("--plain-games", type=int, default=128)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=1000)
-    ap.add_argument("--serve-plain-requests", type=int, default=150)
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-capacity-requests", type=int, default=500)
+    ap.add_argument("--serve-plain-requests", type=int, default=300)
     ap.add_argument("--serve-load", type=float, default=0.75)
     ap.add_argument("--serve-arena-gb", type=float, default=24.0)
-    ap.add_argument("--slice-sweeps", type=int, default=64)
+    ap.add_argument("--slice-kedges", type=int, default=0)
     args = ap.parse_args()
 
     if args.gpus != 1:

@@ -66,7 +66,8 @@ constexpr int kPinViol = 704;     // 1 x u64
 constexpr int kPinCert = 1024;    // certification readbacks (2 x 64 + 16 ints)
 constexpr int kPinForce = 1280;   // round number forced to a full sweep
 constexpr int kCertFirstCap = 64;     // longest-path passes per candidate at the first attempt
-constexpr int kCertCandidates = 8;    // thresholds tried per attempt
+constexpr int kCertCandidates = 8;    // gap thresholds tried per attempt (plus the last-round rise set)
+constexpr int kCertEvery = 16;        // default rounds before the first proof attempt
 
 int pow2ceil(int x) {
     int p = 1;
@@ -88,7 +89,7 @@ struct egs_solver {
     int *d_sbig = nullptr, *d_shuge = nullptr, *d_lcnt = nullptr, *d_mode = nullptr;
     int *d_ring = nullptr, *d_rounds = nullptr, *d_misc = nullptr, *d_tile = nullptr, *d_fill = nullptr;
     int sub_cap = 0;
-    int *d_sigma = nullptr, *d_hist = nullptr, *d_bfcnt = nullptr;
+    int *d_sigma = nullptr, *d_hist = nullptr, *d_bfcnt = nullptr;   // d_sigma: rise stamps
     long long *d_h = nullptr;
     int wmax = 0;
     int cert_every = 32, cert_gap = 32, cert_max_passes = 1024;
@@ -366,8 +367,9 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     s->batch_rounds = o->batch_rounds > 0 ? o->batch_rounds : 32;
     s->phase_cap = egs_max_loop_c(g.m, g.n, g.mg, o->max_phases);
     s->stop_flag = o->stop_flag;
-    s->cert_every = o->certify_every == 0 ? 32 : (o->certify_every < 0 ? 0 : o->certify_every);
+    s->cert_every = o->certify_every == 0 ? kCertEvery : (o->certify_every < 0 ? 0 : o->certify_every);
     s->cert_max_passes = o->cert_max_passes > 0 ? o->cert_max_passes : 1024;
+    g.rise = s->cert_every > 0 ? s->d_sigma : nullptr;    // rise stamps only when proofs run
     return EGS_OK;
 }
 
@@ -437,9 +439,10 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
                                    sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
             HIP_TRY(hipStreamSynchronize(s->stream));
             if (list_total(s, s->h_probe + kPinConv) == 0) break;
-            egs::CertBuffers cb{s->d_hist, s->d_sigma, s->d_h, s->d_bfcnt, s->h_probe + kPinCert};
+            egs::CertBuffers cb{s->d_hist, s->d_h, s->d_bfcnt, s->h_probe + kPinCert};
             egs::CertResult cr;
-            const int got = egs::certify_top(s->g, cb, s->wmax, s->cert_cap, kCertCandidates, s->stream, cr);
+            const int got = egs::certify_top(s->g, cb, s->wmax, (int)(s->next_round - 1), s->cert_cap,
+                                             kCertCandidates, s->stream, cr);
             if (got < 0) return fail(EGS_E_HIP, "certification: %s", hipGetErrorString(hipGetLastError()));
             s->st.cert_attempts++;
             s->st.cert_passes += cr.passes;

@@ -33,7 +33,7 @@ int pack_shift(int n, int wmax) {
 }
 
 int lanes_for(double avg_deg, const egs_batch_options &o) {
-    const int x16 = o.lanes_per_edge_x16 > 0 ? o.lanes_per_edge_x16 : 8;
+    const int x16 = o.lanes_per_edge_x16 > 0 ? o.lanes_per_edge_x16 : 2;
     const int gmax = o.group_max > 0 ? o.group_max : 64;
     const int gmin = o.group_min > 0 ? o.group_min : 1;
     int G = 1;
@@ -167,7 +167,7 @@ int egs_batch_upload(egs_batch *b) {
     batch_free_device(b);
     const size_t ng = b->games.size();
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    // per game: ptr, edges, values, sigma, h
+    // per game: ptr, edges, values, rise stamps, h
     std::vector<size_t> at(ng);
     std::vector<int> shift(ng);
     size_t off = al(sizeof(egs::GameDesc) * ng) + al(sizeof(egs::GameState) * ng) + al(4 * ng) + 256;
@@ -199,7 +199,7 @@ int egs_batch_upload(egs_batch *b) {
         p += al((shift[i] ? 4 : 8) * (size_t)g.m);
         d.values = (int *)p;
         p += al(4 * (size_t)g.n);
-        d.sigma = (int *)p;
+        d.rise = (int *)p;
         p += al(4 * (size_t)g.n);
         d.h = (long long *)p;
         d.state = b->d_states + i;
@@ -254,7 +254,7 @@ int egs_batch_solve(egs_batch *b, const egs_batch_options *o, egs_batch_stats *o
     BHIP(hipMemcpyAsync(b->d_order, order.data(), 4 * (size_t)ng, hipMemcpyHostToDevice, b->stream));
     BHIP(hipMemsetAsync(b->d_counter, 0, 4, b->stream));
     const int64_t max_sweeps = o->max_sweeps > 0 ? o->max_sweeps : INT64_MAX;
-    egs::CertArgs ca{o->certify_every == 0 ? 16 : (o->certify_every < 0 ? 0 : o->certify_every),
+    egs::CertArgs ca{o->certify_every == 0 ? egs::kGameCertifyEvery : (o->certify_every < 0 ? 0 : o->certify_every),
                      o->cert_max_passes > 0 ? o->cert_max_passes : 1024, egs::kGameFirstCap,
                      egs::kGameCandidates};
     BHIP(hipEventRecord(b->ev_a, b->stream));

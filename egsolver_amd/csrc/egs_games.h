@@ -14,6 +14,8 @@
 // proof of egs_accel.hip on its game.
 #pragma once
 
+#include <climits>
+
 #include "egs_device.h"
 
 namespace egs {
@@ -38,8 +40,8 @@ struct GameDesc {
     const int *ptr;        // n+1 local CSR offsets
     const void *edges;     // m edges: uint32 (w << shift) | succ, or int2 {succ, w} when shift == 0
     int *values;           // n
-    int *sigma;            // n scratch (proofs)
-    long long *h;          // n scratch (proofs)
+    int *rise;             // n: number of the sweep in which each vertex last rose (proof candidates)
+    long long *h;          // n scratch (proofs: longest-path potentials)
     GameState *state;
     int64_t m;
     int n, n0, mg, wmax;
@@ -47,7 +49,8 @@ struct GameDesc {
 };
 
 constexpr int kGameFirstCap = 32;      // longest-path passes per candidate at a game's first attempt
-constexpr int kGameCandidates = 8;     // thresholds tried per attempt
+constexpr int kGameCandidates = 8;     // gap thresholds tried per attempt (plus the last-rise set)
+constexpr int kGameCertifyEvery = 8;   // default sweeps before the first proof attempt
 
 struct CertArgs {
     int every;             // sweeps between proof attempts; <= 0 = no proofs
@@ -73,7 +76,7 @@ __device__ __forceinline__ void game_edge(const GameDesc &b, int64_t k, int &suc
 // the chunk's edge count, read from the offsets by every lane, so it stays block-uniform
 // and the loop keeps waves whole for the DPP reductions.
 template <int G>
-__device__ __forceinline__ int game_sweep_part(const GameDesc &b, int *val, int cursor, int64_t budget,
+__device__ __forceinline__ int game_sweep_part(const GameDesc &b, int *val, int cursor, int64_t budget, int sweep_no,
                                                int64_t &spent, int64_t &edges_done, int &lifted, bool &changed) {
     const int gl = threadIdx.x % G;
     const int gid = threadIdx.x / G;
@@ -102,6 +105,7 @@ __device__ __forceinline__ int game_sweep_part(const GameDesc &b, int *val, int 
             edges_done += e - s;
             if (old < t) {
                 val[v] = t;
+                b.rise[v] = sweep_no;
                 changed = true;
                 lifted++;
             }
@@ -111,19 +115,39 @@ __device__ __forceinline__ int game_sweep_part(const GameDesc &b, int *val, int 
     return min(base, b.n);
 }
 
-__device__ __forceinline__ bool game_in_d(int x, int B) { return x > B && x != EGS_TOP; }
+// The certified TOP proof inside the workgroup (the steps of egs_accel.hip).  Candidate
+// sets D, tried in turn until one passes both checks:
+//   * every admissible gap threshold B, smallest first: D = {v : B < f(v) < TOP};
+//   * the vertices that rose in the last sweep (`last`): D = {v : rise(v) == last, f(v) < TOP}.
+// Check 1 (closure): an owner-0 vertex of D has every successor in D or TOP, an owner-1
+// vertex at least one.  Check 2: the longest-path game on D with w' = (n+1) w + 1,
+//   h(v) <- max(h(v), max_{u in D} w'(v,u) + h(u))           (owner 0)
+//   h(v) <- max(h(v), min_{u in D or TOP} w'(v,u) + h(u))    (owner 1; a TOP successor ends the path)
+// reaches a fixpoint within `cap` passes.  At a fixpoint the minimising successors form a
+// strategy sigma of player 1 under which every cycle inside D has weight <= -1, which is
+// the proof; any fixpoint will do, so the iteration starts from h = (n+1) (MG - f): the
+// lifting keeps f(u) - f(v) >= w(v,u) along such edges, so this start is already close.
+// Returns the vertices set to TOP; `cap_hit` tells whether some candidate ran out of passes.
+constexpr long long kGameHOut = LLONG_MIN;       // potential of a vertex outside D
+constexpr long long kGameHTop = LLONG_MIN + 1;   // potential of a TOP vertex
 
-// The certified TOP proof inside the workgroup (the steps of egs_accel.hip): every
-// admissible gap threshold, smallest first, gets the closure check and at most `cap`
-// longest-path passes; the first D that passes both is set to TOP.  Returns the vertices
-// set to TOP; `cap_hit` tells whether some candidate ran out of passes.
-__device__ int game_certify(const GameDesc &b, int *val, int cap, int max_cands, int64_t &passes_done,
+__device__ __forceinline__ bool game_in_d(const GameDesc &b, const int *val, int v, int B, int last) {
+    const int x = val[v];
+    return x != EGS_TOP && (B >= 0 ? x > B : b.rise[v] == last);
+}
+
+__device__ int game_certify(const GameDesc &b, int *val, int last, int cap, int max_cands, int64_t &passes_done,
                             bool &cap_hit) {
     __shared__ int win[64], flo[64];
     __shared__ unsigned long long s_cand;
     __shared__ int s_marked;
     const int wmax = b.wmax;
     if (wmax <= 0) return 0;
+    const long long scale = (long long)b.n + 1;
+    // divergence bound: at a fixpoint h <= max start + (n - 1) max w'; 2x headroom in int64
+    const double bound_d = (double)scale * b.mg + (double)b.n * ((double)scale * wmax + 1);
+    if (bound_d > 4.0e18) return 0;
+    const long long bound = (long long)scale * b.mg + (long long)b.n * (scale * wmax + 1);
     for (int i = threadIdx.x; i < 64; i += blockDim.x) { win[i] = 0; flo[i] = 0; }
     __syncthreads();
     for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
@@ -143,60 +167,76 @@ __device__ int game_certify(const GameDesc &b, int *val, int cap, int max_cands,
             above += flo[j];
             if (((int64_t)wmax << j) <= (int64_t)b.mg - wmax && win[j] == 0 && above > 0) mask |= 1ull << j;
         }
-        s_cand = mask;
+        unsigned long long keep = 0;          // the lowest max_cands thresholds
+        for (int k = 0; k < max_cands && mask; k++) {
+            keep |= mask & (~mask + 1);
+            mask &= mask - 1;
+        }
+        s_cand = keep | 1ull;                 // bit 0: the vertices that rose in the last sweep
         s_marked = 0;
     }
     __syncthreads();
     unsigned long long cand = s_cand;
-    const long long scale = (long long)b.n + 1;
-    const long long bound = (long long)b.n * (scale * wmax + 1);
-    for (int tried = 0; cand && tried < max_cands; tried++) {
-        const int j = __ffsll(cand) - 1;
-        cand &= cand - 1;
-        const int B = (int)((int64_t)wmax << j);
-        bool bad = false;
-        for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
-            b.h[v] = 0;
-            if (!game_in_d(val[v], B)) continue;
-            const int s = b.ptr[v], e = b.ptr[v + 1];
-            if (v >= b.n0) {
-                int best = -1, arg = s;
-                for (int k = s; k < e; k++) {
-                    int u, w;
-                    game_edge(b, k, u, w);
-                    const int c = egs_ominus(val[u], w, b.mg);
-                    if (c > best) { best = c; arg = k; }
-                }
-                b.sigma[v] = arg;
-                int u, w;
-                game_edge(b, arg, u, w);
-                if (val[u] <= B) bad = true;
-            } else {
-                for (int k = s; k < e; k++) {
-                    int u, w;
-                    game_edge(b, k, u, w);
-                    if (val[u] <= B) bad = true;
-                }
-            }
+    while (cand) {
+        // gap thresholds first, smallest first; the last-rise set after them
+        const bool recent = (cand & ~1ull) == 0;
+        int B;
+        if (recent) {
+            B = -1;
+            cand = 0;
+        } else {
+            const int j = __ffsll(cand & ~1ull) - 1;
+            cand &= ~(1ull << j);
+            B = (int)((int64_t)wmax << j);
         }
-        if (__syncthreads_or(bad)) continue;
+        bool bad = false;
+        int members = 0;
+        for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
+            // h >= 0 marks D for the passes; sentinels elsewhere (one 8-byte read per edge)
+            if (val[v] == EGS_TOP) { b.h[v] = kGameHTop; continue; }
+            if (!game_in_d(b, val, v, B, last)) { b.h[v] = kGameHOut; continue; }
+            members++;
+            b.h[v] = scale * ((long long)b.mg - val[v]);
+            const int s = b.ptr[v], e = b.ptr[v + 1];
+            bool any = false, all = true;
+            for (int k = s; k < e; k++) {
+                int u, w;
+                game_edge(b, k, u, w);
+                const bool ok = val[u] == EGS_TOP || game_in_d(b, val, u, B, last);
+                any |= ok;
+                all &= ok;
+            }
+            if (v >= b.n0 ? !any : !all) bad = true;
+        }
+        const int in_set = __syncthreads_count(members > 0);
+        if (__syncthreads_or(bad) || in_set == 0) continue;
         bool converged = false, diverged = false;
         for (int p = 0; p < cap; p++) {
             bool ch = false, over = false;
             for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
-                if (!game_in_d(val[v], B)) continue;
-                long long best = 0;
-                const int s = (v >= b.n0) ? b.sigma[v] : b.ptr[v];
-                const int e = (v >= b.n0) ? b.sigma[v] + 1 : b.ptr[v + 1];
-                for (int k = s; k < e; k++) {
-                    int u, w;
-                    game_edge(b, k, u, w);
-                    if (game_in_d(val[u], B)) {
-                        const long long c = scale * w + 1 + b.h[u];
-                        if (c > best) best = c;
+                const long long hv = b.h[v];
+                if (hv < 0) continue;
+                long long best = hv;
+                const int s = b.ptr[v], e = b.ptr[v + 1];
+                if (v >= b.n0) {
+                    long long mn = LLONG_MAX;
+                    for (int k = s; k < e; k++) {
+                        int u, w;
+                        game_edge(b, k, u, w);
+                        const long long hu = b.h[u];
+                        if (hu == kGameHTop) { mn = LLONG_MIN; break; }
+                        if (hu != kGameHOut) mn = min(mn, scale * w + 1 + hu);
+                    }
+                    if (mn != LLONG_MAX && mn > best) best = mn;
+                } else {
+                    for (int k = s; k < e; k++) {
+                        int u, w;
+                        game_edge(b, k, u, w);
+                        const long long hu = b.h[u];
+                        if (hu >= 0) best = max(best, scale * w + 1 + hu);
                     }
                 }
-                if (best > b.h[v]) {
+                if (best > hv) {
                     b.h[v] = best;
                     ch = true;
                     over |= best > bound;
@@ -213,7 +253,7 @@ __device__ int game_certify(const GameDesc &b, int *val, int cap, int max_cands,
         }
         int mine = 0;
         for (int v = threadIdx.x; v < b.n; v += blockDim.x)
-            if (game_in_d(val[v], B)) { val[v] = EGS_TOP; mine++; }
+            if (b.h[v] >= 0) { val[v] = EGS_TOP; mine++; }
         if (mine) atomicAdd(&s_marked, mine);
         __syncthreads();
         return s_marked;
@@ -240,7 +280,7 @@ __device__ bool game_run(const GameDesc &b, int *val, GameState &sst, int64_t bu
     bool proof_due = false;
     while (!converged && used < budget && done < max_sweeps) {
         bool ch = false;
-        cursor = game_sweep_part<G>(b, val, cursor, budget, used, edges_done, lifted, ch);
+        cursor = game_sweep_part<G>(b, val, cursor, budget, (int)(sweeps0 + done + 1), used, edges_done, lifted, ch);
         dirty = __syncthreads_or(ch || dirty) != 0;
         if (cursor < b.n) break;                    // budget spent part-way through the sweep
         done++;
@@ -282,7 +322,7 @@ __device__ void game_prove(const GameDesc &b, int *val, GameState &sst, const Ce
     __syncthreads();
     int64_t passes = 0;
     bool cap_hit = false;
-    const int marked = game_certify(b, val, cap, ca.candidates, passes, cap_hit);
+    const int marked = game_certify(b, val, (int)sst.sweeps, cap, ca.candidates, passes, cap_hit);
     const bool hit = __syncthreads_or(cap_hit) != 0;
     if (threadIdx.x == 0) {
         sst.cert_passes += passes;
@@ -332,7 +372,10 @@ static __global__ __launch_bounds__(1024) void k_games(const GameDesc *__restric
         __syncthreads();
         const bool fresh = s_idx < 0;
         if (fresh) {
-            for (int v = threadIdx.x; v < b.n; v += blockDim.x) val[v] = 0;
+            for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
+                val[v] = 0;
+                b.rise[v] = 0;
+            }
         } else if (in_lds) {
             for (int v = threadIdx.x; v < b.n; v += blockDim.x) lds_val[v] = b.values[v];
         }

@@ -41,6 +41,8 @@ struct DevGame {
     int64_t dense_min;    // estimated work (vertices) at or above which a round sweeps all
     unsigned long long *stats;  // per-workgroup slots of {processed, edges, lifted, preds}
     int *rounds_done;     // last round that was not MODE_DONE
+    int *rise;            // n: round in which each vertex last rose (TOP proof candidates);
+                          // null when proofs are off
 };
 
 // Launch shape of one round.
@@ -64,8 +66,7 @@ int grid_for(int64_t work, int threads, int cap);
 // Certified TOP detection (egs_accel.hip).
 struct CertBuffers {
     int *hist;        // device: 2 x 64 histogram bins + 8 flags
-    int *sigma;       // device: n
-    long long *h;     // device: n
+    long long *h;     // device: n longest-path potentials
     int *bfcnt;       // device: 8 pass counters
     int *pinned;      // host pinned: 2 x 64 + 16 ints
 };
@@ -79,7 +80,7 @@ struct CertResult {
 };
 hipError_t launch_abs_wmax(const DevGame &g, int *out, hipStream_t st);
 int gap_candidates(const int *win, const int *flo, int wmax, int mg, int64_t *out, int max_out);
-int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int cap, int max_cands, hipStream_t st,
-                CertResult &res);
+int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int cap, int max_cands,
+                hipStream_t st, CertResult &res);
 
 }  // namespace egs

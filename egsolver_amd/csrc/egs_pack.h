@@ -1,6 +1,7 @@
 // egs_pack.h -- host-side CSR checks and edge packing shared by the solver, batch and
 // server entry points (egs_pack.cpp, OpenMP-parallel above a few thousand edges).
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -21,6 +22,9 @@ void egs_pk_pack32(int64_t m, const int32_t *succ, const int32_t *w, int shift, 
 
 // out[2e] = succ[e], out[2e+1] = w[e]   (the int2 edge layout)
 void egs_pk_pack64(int64_t m, const int32_t *succ, const int32_t *w, int32_t *out);
+
+// memcpy split over threads (staging copies of large games)
+void egs_pk_copy(void *dst, const void *src, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -243,6 +243,7 @@ __global__ __launch_bounds__(256) void k_init(DevGame g) {
             act = (v < g.n0) ? all_neg : any_neg;
             g.val[v] = 0;
             g.stamp[v] = act ? 1 : 0;
+            if (g.rise) g.rise[v] = 0;
         }
         stage_push(g, st, act, v);
         stage_flush(g, st, kLdsCap / 2);
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(256) void k_init(DevGame g) {
 // ---------------------------------------------------------------------------------------
 
 template <int G>
-__device__ __forceinline__ void body_dense(const DevGame &g, const Stage &st, LiftAcc &acc) {
+__device__ __forceinline__ void body_dense(const DevGame &g, const Stage &st, int r, LiftAcc &acc) {
     constexpr int ITEMS = 256 / G;
     const int gl = threadIdx.x % G;
     int it = 0;
@@ -274,7 +275,7 @@ __device__ __forceinline__ void body_dense(const DevGame &g, const Stage &st, Li
         if (valid && gl == 0) {
             acc.processed++;
             acc.edges += (uint32_t)(e - s);
-            if (lifted) { g.val[v] = t; acc.lifted++; }
+            if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; }
         }
         stage_push(g, st, lifted && gl == 0, v);
         if ((it & 7) == 7) stage_flush(g, st, kLdsCap / 2);
@@ -309,7 +310,7 @@ __device__ __forceinline__ void body_queue(const DevGame &g, const Stage &st, co
         if (valid && gl == 0) {
             acc.processed++;
             acc.edges += (uint32_t)(e - s);
-            if (lifted) { g.val[v] = t; acc.lifted++; }
+            if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; }
         }
         push_preds<G>(g, st, v, lifted, gl, r, acc);
         if ((it & 3) == 3) stage_flush(g, st, kLdsCap / 2);
@@ -349,6 +350,7 @@ __device__ __forceinline__ void body_lifted(const DevGame &g, const Stage &st, c
                     if (old < t) {
                         g.val[p] = t;
                         acc.lifted++;
+                        if (g.rise) g.rise[p] = r;
                         const int qs = g.cptr[p], qe = g.cptr[p + 1];
                         acc.preds += (uint32_t)(qe - qs);
                         for (int q = qs; q < qe; q++) {
@@ -392,7 +394,7 @@ __global__ __launch_bounds__(256) void k_round(DevGame g, int r) {
     const Stage st{sbuf, &scnt, g.list[(r + 1) & 1], g.lcnt + ((r + 1) % 3) * kListCounters,
                    (int)(blockIdx.x % kSub)};
     LiftAcc acc;
-    if (mode == MODE_DENSE) body_dense<G>(g, st, acc);
+    if (mode == MODE_DENSE) body_dense<G>(g, st, r, acc);
     else if (mode == MODE_QUEUE) body_queue<G>(g, st, g.list[r & 1], pre, r, acc);
     else body_lifted<G>(g, st, g.list[r & 1], pre, r, acc);
     stage_flush(g, st, 1);
@@ -427,7 +429,7 @@ __global__ __launch_bounds__(256) void k_lift_wave(DevGame g, int r) {
         if (lane == 0) {
             acc.processed++;
             acc.edges += (uint32_t)(e - s);
-            if (lifted) { g.val[v] = t; acc.lifted++; }
+            if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; }
         }
         if (dense) stage_push(g, st, lifted && lane == 0, v);
         else push_preds<64>(g, st, v, lifted, lane, r, acc);
@@ -475,7 +477,7 @@ __global__ __launch_bounds__(256) void k_lift_block(DevGame g, int r) {
         if (threadIdx.x == 0) {
             acc.processed++;
             acc.edges += (uint32_t)(e - s);
-            if (lifted) { g.val[v] = t; acc.lifted++; }
+            if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; }
         }
         if (dense) stage_push(g, st, lifted && threadIdx.x == 0, v);
         else push_preds<256>(g, st, v, lifted, (int)threadIdx.x, r, acc);

@@ -5,17 +5,27 @@
 // No reference counterpart.  Each game's values are the least fixpoint the single-game
 // path gives (the slices resume the same computation; see GameState in egs_games.h).
 //
+// Two routes, decided at admission by vertex count:
+//   * small games (values fit a workgroup's LDS share) run in time slices of the
+//     many-games kernel (egs_games.h), a workgroup per game, edge budget per slice;
+//   * large games are solved one after another by the device-wide single-game solver
+//     (egs_api.hip) from their resident arena copy, while a slice runs beside them.
 // Host protocol: submit() packs a game into pinned staging and copies it into its arena
 // chunk on the copy stream (overlapping a running slice); launch() starts one slice over
-// every resident unfinished game on the compute stream after the pending uploads;
-// wait() ends the slice and reports the games that finished; fetch() copies a result
-// out and frees the chunk.
+// every resident unfinished small game on the compute stream after the pending uploads;
+// wait() solves the queued large games (up to a per-call limit), ends the slice and
+// reports the games that finished; fetch() copies a result out and frees the chunk.
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <condition_variable>
+#include <deque>
 #include <map>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/egs.h"
@@ -77,18 +87,42 @@ struct Arena {
 
 struct Slot {
     bool used = false, finished = false, in_slice = false;
+    bool large = false;   // device-wide route
     size_t at = 0, bytes = 0;
-    int n = 0;
+    int n = 0, n0 = 0, mg = 0;
     int64_t m = 0;
     int64_t seq = 0;      // admission order
+    int64_t sweeps = 0, edges = 0;   // large route: rounds and edges relaxed
+    int rc = 0;           // large route: the solve's return code
+    std::string err;      // and its message
+};
+
+constexpr int kLargeWorkers = 2;     // default large-route threads (each with its own solver and stream)
+constexpr int kLargeCus = 64;        // default compute units a slice leaves to the large route
+// Default route threshold: a workgroup per game pays off while the game's values stay in
+// LDS or close to it; from this many vertices the device-wide solver serves it sooner.
+constexpr int kLargeN = 65536;
+
+// One device-wide solver of the large route (grown to the largest game it has seen).
+struct LargeWorker {
+    egs_solver *solver = nullptr;
+    int n = 0;
+    int64_t m = 0;
 };
 
 constexpr int kSliceKedges = 2048;   // default edge budget per workgroup per slice, in thousands
 
+// Arena bytes of a small-route game: offsets, packed edges, values, rise stamps, potentials.
 size_t footprint(int n, int64_t m, int wmax) {
     const int shift = egs::pack_shift(n, wmax);
     return al256(4 * ((size_t)n + 1)) + al256((shift ? 4 : 8) * (size_t)m) + al256(4 * (size_t)n) +
            al256(4 * (size_t)n) + al256(8 * (size_t)n);
+}
+
+// Arena bytes of a large-route game: offsets, successors, weights, values (never more than
+// the small layout, so footprint() bounds both).
+size_t footprint_large(int n, int64_t m) {
+    return al256(4 * ((size_t)n + 1)) + 2 * al256(4 * (size_t)m) + al256(4 * (size_t)n);
 }
 }  // namespace
 
@@ -108,14 +142,189 @@ struct egs_server {
     egs::GameState *states = nullptr;   // asynchronous
     int *order = nullptr;
     std::vector<int> active;
-    void *staging = nullptr;
-    size_t staging_bytes = 0;
+    // Pinned staging ring: a submit packs into the next buffer after that buffer's previous
+    // upload has finished, so consecutive submits do not wait for each other's copies.
+    static constexpr int kStaging = 3;
+    void *staging[kStaging] = {nullptr, nullptr, nullptr};
+    size_t staging_bytes[kStaging] = {0, 0, 0};
+    hipEvent_t staging_ev[kStaging] = {nullptr, nullptr, nullptr};
+    int staging_next = 0;
     int lds_limit = 0, cus = 0;
     bool slice_running = false;
     double last_slice_ms = 0;
     int64_t admitted = 0;     // admissions so far (slot sequence numbers)
     int hi_slot = -1;         // highest slot index ever used (descriptor/state copies cover 0..hi_slot)
+    bool slice_kernel = false;  // the running slice launched the many-games kernel
+    int large_n = 0;          // games with n >= large_n take the device-wide route
+    // Large route: persistent worker threads take games from large_queue, solve them on
+    // their own solver and stream, and hand them back through large_done.  mu guards
+    // large_queue, large_done, busy, stopping and large_opt; a worker writes only its
+    // game's slot result fields, before the hand-off under mu.
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<int> large_queue;        // admitted large games, oldest first
+    std::vector<int> large_done;        // solved, not yet reported by wait()
+    int busy = 0;                       // large games being solved
+    bool stopping = false;
+    egs_serve_options large_opt{};      // proof options of the latest launch()
+    std::vector<LargeWorker> workers;   // one per thread
+    std::vector<std::thread> threads;
+    int large_workers = kLargeWorkers;
 };
+
+namespace {
+
+// Take the next staging buffer with at least `need` bytes, once its previous upload is done;
+// *buf receives its index (record staging_ev[*buf] after enqueueing the upload from it).
+int staging_reserve(egs_server *s, size_t need, int *buf) {
+    const int i = s->staging_next;
+    s->staging_next = (i + 1) % egs_server::kStaging;
+    SHIP(hipEventSynchronize(s->staging_ev[i]));
+    *buf = i;
+    if (s->staging_bytes[i] >= need) return EGS_OK;
+    if (s->staging[i]) (void)hipHostFree(s->staging[i]);
+    s->staging_bytes[i] = need + need / 2;
+    if (hipHostMalloc(&s->staging[i], s->staging_bytes[i], hipHostMallocDefault) != hipSuccess) {
+        s->staging[i] = nullptr;
+        s->staging_bytes[i] = 0;
+        return sfail(EGS_E_NOMEM, "pinned staging allocation failed");
+    }
+    return EGS_OK;
+}
+
+int solve_large(egs_server *s, LargeWorker &wk, int ticket, const egs_serve_options *o);
+
+// Worker thread of the large route: until the server stops, take the oldest queued game,
+// wait for the uploads, solve it, hand it back.
+void large_worker_main(egs_server *s, int wi) {
+    (void)hipSetDevice(s->device);
+    LargeWorker &wk = s->workers[wi];
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_work.wait(lk, [&] { return s->stopping || !s->large_queue.empty(); });
+        if (s->stopping) break;
+        const int t = s->large_queue.front();
+        s->large_queue.pop_front();
+        const egs_serve_options o = s->large_opt;
+        s->busy++;
+        lk.unlock();
+        int rc = hipStreamSynchronize(s->copy) == hipSuccess ? EGS_OK : sfail(EGS_E_HIP, "upload wait failed");
+        if (!rc) rc = solve_large(s, wk, t, &o);
+        Slot &sl = s->slots[t];
+        sl.rc = rc;
+        sl.err = rc ? s_err : "";
+        lk.lock();
+        s->busy--;
+        s->large_done.push_back(t);
+        s->cv_done.notify_all();
+    }
+}
+
+void start_workers(egs_server *s) {
+    if (!s->threads.empty()) return;
+    s->workers.assign(s->large_workers, LargeWorker{});
+    for (int wi = 0; wi < s->large_workers; wi++) s->threads.emplace_back(large_worker_main, s, wi);
+}
+
+void stop_workers(egs_server *s) {
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->threads) t.join();
+    s->threads.clear();
+    for (auto &w : s->workers)
+        if (w.solver) egs_solver_destroy(w.solver);
+    s->workers.clear();
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = false;
+}
+
+// Large route: the raw CSR goes into the arena chunk; the game waits in large_queue.
+int submit_large(egs_server *s, int ticket, size_t at, size_t bytes, int32_t n, int32_t n0, int64_t m, int32_t mg,
+                 const int32_t *ptr, const int32_t *succ, const int32_t *w) {
+    const size_t ptr_b = 4 * ((size_t)n + 1), e_b = 4 * (size_t)m;
+    const size_t o_succ = al256(ptr_b), o_w = o_succ + al256(e_b), o_val = o_w + al256(e_b);
+    int sb = 0;
+    int rc = staging_reserve(s, o_val, &sb);
+    if (rc) {
+        s->alloc.release(at, bytes);
+        return rc;
+    }
+    char *stg = (char *)s->staging[sb];
+    egs_pk_copy(stg, ptr, ptr_b);
+    egs_pk_copy(stg + o_succ, succ, e_b);
+    egs_pk_copy(stg + o_w, w, e_b);
+    char *base = (char *)s->arena + at;
+    SHIP(hipMemcpyAsync(base, stg, o_val, hipMemcpyHostToDevice, s->copy));
+    SHIP(hipEventRecord(s->staging_ev[sb], s->copy));
+    SHIP(hipEventRecord(s->ev_upload, s->copy));
+    Slot &sl = s->slots[ticket];
+    sl = Slot{};
+    sl.used = true;
+    sl.large = true;
+    sl.at = at;
+    sl.bytes = bytes;
+    sl.n = n;
+    sl.n0 = n0;
+    sl.mg = mg;
+    sl.m = m;
+    sl.seq = s->admitted++;
+    egs::GameDesc d{};
+    d.ptr = (const int *)base;
+    d.values = (int *)(base + o_val);
+    d.n = n;
+    d.m = m;
+    s->descs[ticket] = d;
+    start_workers(s);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->large_queue.push_back(ticket);
+    }
+    s->cv_work.notify_one();
+    return ticket;
+}
+
+// Solve one queued large game with a worker's device-wide solver (on the worker's own
+// stream and host thread); its values go to its arena chunk.  Uploads are complete (the
+// caller synchronised the copy stream).
+int solve_large(egs_server *s, LargeWorker &wk, int ticket, const egs_serve_options *o) {
+    Slot &sl = s->slots[ticket];
+    if (!wk.solver || wk.n < sl.n || wk.m < sl.m) {
+        if (wk.solver) egs_solver_destroy(wk.solver);
+        wk.solver = nullptr;
+        int nc = 1024;
+        while (nc < sl.n) nc *= 2;
+        int64_t mc = 4096;
+        while (mc < sl.m) mc *= 2;
+        int rc = egs_solver_create(s->device, nc, mc, &wk.solver);
+        if (rc) return sfail(rc, "large-route solver: %s", egs_last_error());
+        wk.n = nc;
+        wk.m = mc;
+    }
+    const char *base = (const char *)s->arena + sl.at;
+    const size_t o_succ = al256(4 * ((size_t)sl.n + 1)), o_w = o_succ + al256(4 * (size_t)sl.m);
+    int rc = egs_solver_load(wk.solver, EGS_MEM_DEVICE, sl.n, sl.n0, sl.m, sl.mg, (const int32_t *)base,
+                             (const int32_t *)(base + o_succ), (const int32_t *)(base + o_w));
+    egs_options eo{};
+    eo.certify_every = o->certify_every;
+    eo.cert_max_passes = o->cert_max_passes;
+    eo.max_phases = -1;
+    if (!rc) rc = egs_solver_prepare(wk.solver, &eo);
+    egs_stats st{};
+    if (!rc) rc = egs_solver_iterate(wk.solver, -1, &st);
+    if (rc < 0) return sfail(rc, "large-route solve: %s", egs_last_error());
+    hipStream_t ss = (hipStream_t)egs_solver_stream(wk.solver);
+    SHIP(hipMemcpyAsync(s->descs[ticket].values, egs_solver_device_values(wk.solver), 4 * (size_t)sl.n,
+                        hipMemcpyDeviceToDevice, ss));
+    SHIP(hipStreamSynchronize(ss));
+    sl.sweeps = st.phases;
+    sl.edges = st.edges_relaxed;
+    return EGS_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -135,6 +344,7 @@ int egs_server_create(int device, int64_t arena_bytes, int32_t max_games, egs_se
     s->device = device;
     s->lds_limit = (int)p.sharedMemPerBlock;
     s->cus = p.multiProcessorCount;
+    s->large_n = kLargeN;
     s->max_games = max_games;
     if (hipMalloc(&s->arena, (size_t)arena_bytes) != hipSuccess) {
         delete s;
@@ -167,6 +377,9 @@ int egs_server_create(int device, int64_t arena_bytes, int32_t max_games, egs_se
         hipEventCreate(&s->ev_slice) != hipSuccess || hipEventCreate(&s->ev_a) != hipSuccess) {
         return sfail(EGS_E_HIP, "stream/event creation failed");
     }
+    for (int i = 0; i < egs_server::kStaging; i++)
+        if (hipEventCreateWithFlags(&s->staging_ev[i], hipEventDisableTiming) != hipSuccess)
+            return sfail(EGS_E_HIP, "event creation failed");
     *out = s;
     return EGS_OK;
 }
@@ -176,9 +389,13 @@ int egs_server_destroy(egs_server *s) {
     (void)hipSetDevice(s->device);
     if (s->compute) (void)hipStreamSynchronize(s->compute);
     if (s->copy) (void)hipStreamSynchronize(s->copy);
+    stop_workers(s);
     if (s->arena) (void)hipFree(s->arena);
     if (s->d_small) (void)hipFree(s->d_small);
-    if (s->staging) (void)hipHostFree(s->staging);
+    for (int i = 0; i < egs_server::kStaging; i++) {
+        if (s->staging[i]) (void)hipHostFree(s->staging[i]);
+        if (s->staging_ev[i]) (void)hipEventDestroy(s->staging_ev[i]);
+    }
     if (s->descs) (void)hipHostFree(s->descs);
     if (s->states) (void)hipHostFree(s->states);
     if (s->order) (void)hipHostFree(s->order);
@@ -199,6 +416,23 @@ int egs_server_free_slots(egs_server *s) {
 
 int64_t egs_server_free_bytes(egs_server *s) { return s ? (int64_t)(s->alloc.total - s->alloc.in_use) : 0; }
 
+int egs_server_set_route(egs_server *s, int32_t large_n, int32_t large_workers) {
+    if (!s || large_n < 0 || large_workers < 0 || large_workers > 16)
+        return sfail(EGS_E_ARG, "egs_server_set_route: bad arguments");
+    const int nw = large_workers > 0 ? large_workers : kLargeWorkers;
+    if (nw != s->large_workers && !s->threads.empty()) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!s->large_queue.empty() || s->busy)
+            return sfail(EGS_E_STATE, "egs_server_set_route: large games are in flight");
+    }
+    s->large_n = large_n > 0 ? large_n : kLargeN;
+    if (nw != s->large_workers) {
+        stop_workers(s);                       // restarted with the new count at the next large game
+        s->large_workers = nw;
+    }
+    return EGS_OK;
+}
+
 // Admit a game: returns its ticket, or EGS_E_CAPACITY when the arena or the slot table is
 // full right now (the caller retries after fetching finished games).
 int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *ptr,
@@ -218,27 +452,22 @@ int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t m
     for (int i = 0; i < s->max_games; i++)
         if (!s->slots[i].used) { ticket = i; break; }
     if (ticket < 0) return sfail(EGS_E_CAPACITY, "no free game slot");
-    const size_t bytes = footprint(n, m, wm);
+    const bool large = n >= s->large_n;
+    const size_t bytes = large ? footprint_large(n, m) : footprint(n, m, wm);
     size_t at = 0;
     if (!s->alloc.alloc(bytes, at)) return sfail(EGS_E_CAPACITY, "arena full (%zu bytes wanted)", bytes);
     SHIP(hipSetDevice(s->device));
+    if (large) return submit_large(s, ticket, at, bytes, n, n0, m, mg, ptr, succ, w);
     const int shift = egs::pack_shift(n, wm);
     const size_t ptr_b = 4 * ((size_t)n + 1), edge_b = (shift ? 4 : 8) * (size_t)m;
-    if (s->staging_bytes < ptr_b + edge_b) {
-        SHIP(hipStreamSynchronize(s->copy));
-        if (s->staging) (void)hipHostFree(s->staging);
-        s->staging_bytes = (ptr_b + edge_b) * 2;
-        if (hipHostMalloc(&s->staging, s->staging_bytes, hipHostMallocDefault) != hipSuccess) {
-            s->staging = nullptr;
-            s->staging_bytes = 0;
-            s->alloc.release(at, bytes);
-            return sfail(EGS_E_NOMEM, "pinned staging allocation failed");
-        }
+    int sb = 0;
+    int rc = staging_reserve(s, ptr_b + edge_b, &sb);
+    if (rc) {
+        s->alloc.release(at, bytes);
+        return rc;
     }
-    // the previous upload must be done before the staging buffer is overwritten
-    SHIP(hipStreamSynchronize(s->copy));
-    char *stg = (char *)s->staging;
-    memcpy(stg, ptr, ptr_b);
+    char *stg = (char *)s->staging[sb];
+    egs_pk_copy(stg, ptr, ptr_b);
     if (shift)
         egs_pk_pack32(m, succ, w, shift, (uint32_t *)(stg + ptr_b));
     else
@@ -248,8 +477,8 @@ int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t m
     d.ptr = (const int *)base;
     d.edges = base + al256(ptr_b);
     d.values = (int *)(base + al256(ptr_b) + al256(edge_b));
-    d.sigma = (int *)((char *)d.values + al256(4 * (size_t)n));
-    d.h = (long long *)((char *)d.sigma + al256(4 * (size_t)n));
+    d.rise = (int *)((char *)d.values + al256(4 * (size_t)n));
+    d.h = (long long *)((char *)d.rise + al256(4 * (size_t)n));
     d.state = s->d_states + ticket;
     d.m = m;
     d.n = n;
@@ -260,12 +489,12 @@ int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t m
     d.group = 1;
     SHIP(hipMemcpyAsync((void *)d.ptr, stg, ptr_b, hipMemcpyHostToDevice, s->copy));
     SHIP(hipMemcpyAsync((void *)d.edges, stg + ptr_b, edge_b, hipMemcpyHostToDevice, s->copy));
+    SHIP(hipEventRecord(s->staging_ev[sb], s->copy));
     SHIP(hipMemsetAsync(s->d_states + ticket, 0, sizeof(egs::GameState), s->copy));
     SHIP(hipEventRecord(s->ev_upload, s->copy));
     Slot &sl = s->slots[ticket];
+    sl = Slot{};
     sl.used = true;
-    sl.finished = false;
-    sl.in_slice = false;
     sl.at = at;
     sl.bytes = bytes;
     sl.n = n;
@@ -289,15 +518,26 @@ int egs_server_launch(egs_server *s, const egs_serve_options *o) {
     bo.group_min = o->group_min;
     bo.group_max = o->group_max;
     s->active.clear();
-    for (int i = 0; i < s->max_games; i++) {
+    for (int i = 0; i <= s->hi_slot; i++) {
         Slot &sl = s->slots[i];
-        sl.in_slice = sl.used && !sl.finished;
+        sl.in_slice = sl.used && !sl.finished && !sl.large;
         if (sl.in_slice) {
             s->descs[i].group = egs::lanes_for((double)sl.m / sl.n, bo);
             s->active.push_back(i);
         }
     }
-    if (s->active.empty()) return 0;
+    int large_in_flight;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->large_opt = *o;
+        large_in_flight = (int)s->large_queue.size() + s->busy + (int)s->large_done.size();
+    }
+    if (s->active.empty()) {
+        // nothing for the many-games kernel; wait() still reports the large route
+        s->slice_kernel = false;
+        s->slice_running = large_in_flight > 0;
+        return large_in_flight;
+    }
     // oldest admission first: every workgroup's edge budget bounds the slice, so the games
     // at the front of the order progress every slice and none waits behind a large one
     std::stable_sort(s->active.begin(), s->active.end(),
@@ -306,10 +546,12 @@ int egs_server_launch(egs_server *s, const egs_serve_options *o) {
     if (threads != 256 && threads != 512 && threads != 1024) return sfail(EGS_E_ARG, "threads must be 256, 512 or 1024");
     int lds_ints = o->lds_ints > 0 ? o->lds_ints : 1 << 30;   // default: all the LDS a workgroup may hold
     lds_ints = std::min(lds_ints, (s->lds_limit - 2048) / 4);
-    const int grid = std::max(1, s->cus * (o->blocks_per_cu > 0 ? o->blocks_per_cu : 1));
+    // while large-route games are in flight the slice leaves compute units to their kernels
+    const int free_cus = large_in_flight == 0 ? 0 : o->large_cus < 0 ? 0 : o->large_cus > 0 ? o->large_cus : kLargeCus;
+    const int grid = std::max(1, std::max(1, s->cus - free_cus) * (o->blocks_per_cu > 0 ? o->blocks_per_cu : 1));
     const int64_t max_sweeps = o->slice_sweeps > 0 ? o->slice_sweeps : INT64_MAX;
     const int64_t wg_budget = 1000 * (int64_t)(o->slice_kedges > 0 ? o->slice_kedges : kSliceKedges);
-    egs::CertArgs ca{o->certify_every == 0 ? 16 : (o->certify_every < 0 ? 0 : o->certify_every),
+    egs::CertArgs ca{o->certify_every == 0 ? egs::kGameCertifyEvery : (o->certify_every < 0 ? 0 : o->certify_every),
                      o->cert_max_passes > 0 ? o->cert_max_passes : 1024, egs::kGameFirstCap,
                      egs::kGameCandidates};
     // descriptors of the slice's games, then the uploads, then the kernel
@@ -329,28 +571,60 @@ int egs_server_launch(egs_server *s, const egs_serve_options *o) {
                         hipMemcpyDeviceToHost, s->compute));
     SHIP(hipEventRecord(s->ev_slice, s->compute));
     s->slice_running = true;
-    return (int)s->active.size();
+    s->slice_kernel = true;
+    return (int)s->active.size() + large_in_flight;
 }
 
 // Finish the running slice; writes the tickets that reached their fixpoint.
 int egs_server_wait(egs_server *s, int32_t *done, int32_t cap) {
     if (!s) return sfail(EGS_E_ARG, "null server");
     if (!s->slice_running) return 0;
-    SHIP(hipEventSynchronize(s->ev_slice));
-    float ms = 0.f;
-    SHIP(hipEventElapsedTime(&ms, s->ev_a, s->ev_slice));
-    s->last_slice_ms = ms;
-    s->slice_running = false;
     int k = 0;
-    for (int i : s->active) {
-        Slot &sl = s->slots[i];
-        if (s->states[i].converged && !sl.finished) {
-            sl.finished = true;
-            if (done && k < cap) done[k] = i;
-            k++;
+    s->last_slice_ms = 0;
+    if (s->slice_kernel) {
+        SHIP(hipEventSynchronize(s->ev_slice));
+        float ms = 0.f;
+        SHIP(hipEventElapsedTime(&ms, s->ev_a, s->ev_slice));
+        s->last_slice_ms = ms;
+        for (int i : s->active) {
+            Slot &sl = s->slots[i];
+            if (s->states[i].converged && !sl.finished) {
+                sl.finished = true;
+                if (done && k < cap) done[k] = i;
+                k++;
+            }
         }
     }
+    // large-route completions; with no slice to wait for, block until one arrives
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (k == 0 && !s->slice_kernel)
+            s->cv_done.wait(lk, [&] { return !s->large_done.empty() || (s->large_queue.empty() && s->busy == 0); });
+        for (int t : s->large_done) {
+            s->slots[t].finished = true;
+            if (done && k < cap) done[k] = t;
+            k++;
+        }
+        s->large_done.clear();
+    }
+    s->slice_running = false;
+    s->slice_kernel = false;
     return k;
+}
+
+// 1 when wait() would return without blocking (no slice running, the slice's kernel and
+// copies are done, or a large-route game is ready to report), else 0.
+int egs_server_ready(egs_server *s) {
+    if (!s) return sfail(EGS_E_ARG, "null server");
+    if (!s->slice_running) return 1;
+    if (s->slice_kernel) {
+        const hipError_t q = hipEventQuery(s->ev_slice);
+        if (q == hipErrorNotReady) return 0;
+        if (q != hipSuccess) return sfail(EGS_E_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+        return 1;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    return !s->large_done.empty() || (s->large_queue.empty() && s->busy == 0) ? 1 : 0;
 }
 
 double egs_server_last_slice_ms(egs_server *s) { return s ? s->last_slice_ms : 0.0; }
@@ -366,10 +640,17 @@ int egs_server_fetch(egs_server *s, int ticket, int32_t *values, int64_t *sweeps
     if (s->slice_running) return sfail(EGS_E_STATE, "wait for the running slice first");
     Slot &sl = s->slots[ticket];
     if (!sl.finished) return sfail(EGS_E_STATE, "game %d has not finished", ticket);
+    if (sl.large && sl.rc) {
+        const int rc = sl.rc;
+        const std::string msg = sl.err;
+        s->alloc.release(sl.at, sl.bytes);
+        sl = Slot{};
+        return sfail(rc, "game %d: %s", ticket, msg.c_str());
+    }
     SHIP(hipSetDevice(s->device));
     if (values) SHIP(hipMemcpy(values, s->descs[ticket].values, 4 * (size_t)sl.n, hipMemcpyDeviceToHost));
-    if (sweeps) *sweeps = s->states[ticket].sweeps;
-    if (edges) *edges = s->states[ticket].edges;
+    if (sweeps) *sweeps = sl.large ? sl.sweeps : s->states[ticket].sweeps;
+    if (edges) *edges = sl.large ? sl.edges : s->states[ticket].edges;
     s->alloc.release(sl.at, sl.bytes);
     sl = Slot{};
     return EGS_OK;

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 
 namespace {
 constexpr int64_t kParallelMin = 1 << 15;   // below this many edges one thread is faster
@@ -49,5 +50,15 @@ extern "C" void egs_pk_pack64(int64_t m, const int32_t *succ, const int32_t *w, 
     for (int64_t e = 0; e < m; e++) {
         out[2 * e] = succ[e];
         out[2 * e + 1] = w[e];
+    }
+}
+
+extern "C" void egs_pk_copy(void *dst, const void *src, size_t bytes) {
+    constexpr size_t kChunk = 1 << 20;
+    const int64_t chunks = (int64_t)((bytes + kChunk - 1) / kChunk);
+#pragma omp parallel for if (bytes > 4 * kChunk) schedule(static)
+    for (int64_t c = 0; c < chunks; c++) {
+        const size_t at = (size_t)c * kChunk;
+        memcpy((char *)dst + at, (const char *)src + at, std::min(kChunk, bytes - at));
     }
 }

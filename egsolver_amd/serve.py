@@ -33,6 +33,7 @@ class ServeOptions(ctypes.Structure):
         ("certify_every", ctypes.c_int32),
         ("cert_max_passes", ctypes.c_int32),
         ("slice_kedges", ctypes.c_int32),
+        ("large_cus", ctypes.c_int32),
     ]
 
 
@@ -42,11 +43,13 @@ N.register_signatures({
     "egs_server_footprint": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]),
     "egs_server_free_bytes": (ctypes.c_int64, [_vp]),
     "egs_server_free_slots": (ctypes.c_int, [_vp]),
+    "egs_server_set_route": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
     "egs_server_submit": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                          _vp, _vp, _vp]),
     "egs_server_launch": (ctypes.c_int, [_vp, ctypes.POINTER(ServeOptions)]),
     "egs_server_wait": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
     "egs_server_last_slice_ms": (ctypes.c_double, [_vp]),
+    "egs_server_ready": (ctypes.c_int, [_vp]),
     "egs_server_status": (ctypes.c_int, [_vp, ctypes.c_int]),
     "egs_server_fetch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_int64),
                                         ctypes.POINTER(ctypes.c_int64)]),
@@ -64,9 +67,9 @@ def _check(rc, what):
 
 
 def serve_options(threads=0, blocks_per_cu=0, lds_ints=0, lanes_per_edge_x16=0, group_min=0, group_max=0,
-                  slice_sweeps=0, certify_every=0, cert_max_passes=0, slice_kedges=0):
+                  slice_sweeps=0, certify_every=0, cert_max_passes=0, slice_kedges=0, large_cus=0):
     return ServeOptions(threads, blocks_per_cu, lds_ints, lanes_per_edge_x16, group_min, group_max, slice_sweeps,
-                        certify_every, cert_max_passes, slice_kedges)
+                        certify_every, cert_max_passes, slice_kedges, large_cus)
 
 
 class Server:
@@ -122,6 +125,10 @@ class Server:
         k = _check(N.egs().egs_server_wait(self.h, self._done, self.max_games), "egs_server_wait")
         return [self._done[i] for i in range(min(k, self.max_games))]
 
+    def ready(self):
+        """True when wait() would return without blocking."""
+        return bool(_check(N.egs().egs_server_ready(self.h), "egs_server_ready"))
+
     def last_slice_ms(self):
         return N.egs().egs_server_last_slice_ms(self.h)
 
@@ -136,6 +143,11 @@ class Server:
     def free_bytes(self):
         return int(N.egs().egs_server_free_bytes(self.h))
 
+    def set_route(self, large_n=0, large_workers=0):
+        """Games with at least `large_n` vertices go to the device-wide route (0 = default,
+        65536), solved `large_workers` at a time (0 = default, 2)."""
+        _check(N.egs().egs_server_set_route(self.h, int(large_n), int(large_workers)), "egs_server_set_route")
+
 
 @dataclass
 class Request:
@@ -147,26 +159,38 @@ class Request:
     values: np.ndarray = field(default=None, repr=False)
 
 
-def run_served(requests, arena_bytes, opt=None, max_games=4096, keep_values=False, device=0):
+def run_served(requests, arena_bytes, opt=None, max_games=4096, keep_values=False, device=0, large_n=0,
+               large_workers=0, admit_batch=32):
     """Replay `requests` (sorted by arrival) against a Server; returns (requests,
-    wall seconds, slice count).  Latency of a request = finished - arrival."""
+    wall seconds, slice count).  Latency of a request = finished - arrival.
+
+    Loop: admit up to `admit_batch` arrived requests, start a slice, keep admitting arrived
+    requests while the slice runs, then collect what finished."""
     srv = Server(arena_bytes, max_games=max_games, device=device)
+    srv.set_route(large_n, large_workers)
     waiting = deque(requests)
     running = {}
     slices = 0
     t0 = time.perf_counter()
+
+    def admit(limit, while_busy=False):
+        k = 0
+        while waiting and k < limit and waiting[0].arrival <= time.perf_counter() - t0:
+            if while_busy and srv.ready():
+                break
+            r = waiting[0]
+            t = srv.submit(r.game)
+            if t is None:
+                break
+            r.admitted = time.perf_counter() - t0
+            running[t] = r
+            waiting.popleft()
+            k += 1
+        return k
+
     try:
         while waiting or running:
-            now = time.perf_counter() - t0
-            # admit every arrived request that fits
-            while waiting and waiting[0].arrival <= now:
-                r = waiting[0]
-                t = srv.submit(r.game)
-                if t is None:
-                    break
-                r.admitted = time.perf_counter() - t0
-                running[t] = r
-                waiting.popleft()
+            admit(admit_batch)
             if not running:
                 # idle until the next arrival
                 if waiting:
@@ -174,16 +198,8 @@ def run_served(requests, arena_bytes, opt=None, max_games=4096, keep_values=Fals
                 continue
             srv.launch(opt)
             slices += 1
-            # admissions during the slice overlap its execution
-            now = time.perf_counter() - t0
-            while waiting and waiting[0].arrival <= now:
-                r = waiting[0]
-                t = srv.submit(r.game)
-                if t is None:
-                    break
-                r.admitted = time.perf_counter() - t0
-                running[t] = r
-                waiting.popleft()
+            # admissions while the slice runs overlap its execution
+            admit(max_games, while_busy=True)
             done = srv.wait()
             tf = time.perf_counter() - t0
             for t in done:

@@ -48,8 +48,10 @@ typedef struct egs_options {
                                0 = auto (12), > 100 = never sweep */
     int32_t certify_every;  /* rounds between attempts to prove climbing vertices TOP
                                (exact; attempts back off while they fail);
-                               0 = auto (32), < 0 = never */
-    int32_t cert_max_passes;/* longest-path passes per attempt; 0 = auto (1024) */
+                               0 = auto (16), < 0 = never */
+    int32_t cert_max_passes;/* ceiling of the longest-path passes per proof candidate
+                               (the cap starts at 64 and doubles while candidates run
+                               out of passes); 0 = auto (1024) */
     int64_t max_phases;     /* phase cap; < 0 = the reference bound n*m*MG+1
                                (src/egsolver.c:64-90) */
     const volatile uint32_t *stop_flag;  /* polled between batches; non-zero stops the
@@ -157,10 +159,10 @@ typedef struct egs_batch_options {
     int32_t lds_ints;           /* LDS values per workgroup: games up to this many vertices
                                    keep their values in LDS; 0 = all a workgroup may hold */
     int32_t lanes_per_edge_x16; /* lanes per vertex = smallest power of two >= average
-                                   degree * this / 16; 0 = 8 */
+                                   degree * this / 16; 0 = 2 */
     int32_t group_min, group_max; /* clamp of the lanes per vertex; 0 = 1 / 64 */
     int64_t max_sweeps;         /* per-game sweep cap; <= 0 = none */
-    int32_t certify_every;      /* sweeps between TOP proofs per game; 0 = 16, < 0 = never */
+    int32_t certify_every;      /* sweeps between TOP proofs per game; 0 = 8, < 0 = never */
     int32_t cert_max_passes;    /* ceiling of the longest-path passes per proof candidate; 0 = 1024 */
 } egs_batch_options;
 
@@ -206,14 +208,17 @@ typedef struct egs_serve_options {
     int32_t threads;            /* 256, 512 or 1024; 0 = 1024 */
     int32_t blocks_per_cu;      /* 0 = 1 */
     int32_t lds_ints;           /* 0 = all a workgroup may hold */
-    int32_t lanes_per_edge_x16; /* 0 = 8 */
+    int32_t lanes_per_edge_x16; /* 0 = 2 */
     int32_t group_min, group_max;
     int32_t slice_sweeps;       /* cap of complete sweeps per game per slice; 0 = none */
-    int32_t certify_every;      /* 0 = 16, < 0 = never */
+    int32_t certify_every;      /* 0 = 8, < 0 = never */
     int32_t cert_max_passes;    /* 0 = 1024 */
     int32_t slice_kedges;       /* edge budget of each workgroup per slice, in thousands of
                                    edges (bounds the slice time; sweeps resume part-way);
                                    0 = 2048 */
+    int32_t large_cus;          /* compute units a slice leaves free while large-route games
+                                   are in flight (the slice runs one workgroup per remaining
+                                   CU); 0 = 64, < 0 = none */
 } egs_serve_options;
 
 int egs_server_create(int device, int64_t arena_bytes, int32_t max_games, egs_server **out);
@@ -221,6 +226,11 @@ int egs_server_destroy(egs_server *s);
 /* Device bytes a game of n vertices, m edges and max |w| occupies in the arena. */
 int64_t egs_server_footprint(int32_t n, int64_t m, int32_t wmax);
 int64_t egs_server_free_bytes(egs_server *s);
+/* Games with at least large_n vertices take the large route: wait() solves them with the
+ * device-wide single-game solver, large_workers at a time (host threads, each with its
+ * own solver and stream), beside the running slice, instead of one workgroup each.
+ * large_n 0 = default (65536); large_workers 0 = default (2), at most 16. */
+int egs_server_set_route(egs_server *s, int32_t large_n, int32_t large_workers);
 int egs_server_free_slots(egs_server *s);
 /* Returns a ticket >= 0, or EGS_E_CAPACITY when the game does not fit right now. */
 int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *ptr,
@@ -229,6 +239,9 @@ int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t m
 int egs_server_launch(egs_server *s, const egs_serve_options *opt);
 /* Returns how many games finished in the slice; their tickets go to done[0..cap). */
 int egs_server_wait(egs_server *s, int32_t *done, int32_t cap);
+/* 1 when wait() would return without blocking, else 0 (lets a caller keep admitting
+ * requests while a slice runs). */
+int egs_server_ready(egs_server *s);
 double egs_server_last_slice_ms(egs_server *s);
 int egs_server_status(egs_server *s, int ticket);
 int egs_server_fetch(egs_server *s, int ticket, int32_t *values, int64_t *sweeps, int64_t *edges);

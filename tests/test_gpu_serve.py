@@ -79,3 +79,29 @@ def test_served_partial_sweeps_exact(oracle):
     assert slices > 10
     for r in done:
         assert np.array_equal(r.values, lfp(oracle, to_oracle_game(r.game))), (r.rid, r.game.n)
+
+
+def test_served_large_route_exact(oracle):
+    """Games at or above the route threshold are solved by the device-wide solver beside the
+    running slices; both routes end on the oracle's least fixpoint and on the single-game
+    solver's result."""
+    import egsolver_amd as E
+    reqs = _requests(8, 16, 3000, 0.001)
+    cut = 1500
+    sizes = [r.game.n for r in reqs]
+    assert any(n >= cut for n in sizes) and any(n < cut for n in sizes)
+    _say(f"serve two routes, cut {cut}, sizes {sizes}")
+    arena = sum(E.Server.footprint(r.game) for r in reqs) + (1 << 20)
+    done, wall, slices = E.run_served(reqs, arena, E.serve_options(slice_kedges=64), max_games=32,
+                                      keep_values=True, large_n=cut)
+    for r in done:
+        ref = lfp(oracle, to_oracle_game(r.game))
+        assert np.array_equal(r.values, ref), (r.rid, r.game.n)
+    with E.Server(arena, max_games=4) as srv:
+        srv.set_route(1)                                  # every game on the large route
+        t = srv.submit(reqs[0].game)
+        srv.launch()
+        assert t in srv.wait()
+        vals, st = srv.fetch(t)
+        assert np.array_equal(vals, lfp(oracle, to_oracle_game(reqs[0].game)))
+        assert st["sweeps"] >= 1
