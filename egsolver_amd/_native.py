@@ -42,6 +42,8 @@ class EgsOptions(ctypes.Structure):
         ("cert_max_passes", ctypes.c_int32),
         ("max_phases", ctypes.c_int64),
         ("stop_flag", ctypes.c_void_p),
+        ("slices", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

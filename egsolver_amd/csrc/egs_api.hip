@@ -65,6 +65,8 @@ constexpr int kPinStats = 640;    // 4 x u64
 constexpr int kPinViol = 704;     // 1 x u64
 constexpr int kPinCert = 1024;    // certification readbacks (2 x 64 + 16 ints)
 constexpr int kPinForce = 1280;   // round number forced to a full sweep
+constexpr int kPinOne = 1281;     // the constant 1 (raises ring[12])
+constexpr int kProbeWant = 100;   // offset in a probe slot: ring[13] (a round wanted to go sparse)
 constexpr int kCertFirstCap = 64;     // longest-path passes per candidate at the first attempt
 constexpr int kCertCandidates = 8;    // gap thresholds tried per attempt (plus the last-round rise set)
 constexpr int kCertEvery = 16;        // default rounds before the first proof attempt
@@ -80,6 +82,7 @@ int pow2ceil(int x) {
 struct egs_solver {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool csc_built = false;           // the transpose exists for the loaded game (ring[12] raised)
     int32_t n_cap = 0;
     int64_t m_cap = 0;
     void *block = nullptr;
@@ -90,6 +93,8 @@ struct egs_solver {
     int *d_ring = nullptr, *d_rounds = nullptr, *d_misc = nullptr, *d_tile = nullptr, *d_fill = nullptr;
     int sub_cap = 0;
     int *d_sigma = nullptr, *d_hist = nullptr, *d_bfcnt = nullptr;   // d_sigma: rise stamps
+    int *d_sptr = nullptr, *d_acc = nullptr, *d_stile = nullptr;    // phased dense rounds (large n_cap)
+    int2 *d_sedges = nullptr;
     long long *d_h = nullptr;
     int wmax = 0;
     int cert_every = 32, cert_gap = 32, cert_max_passes = 1024;
@@ -219,6 +224,11 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
            o_tile = take(4 * (size_t)tiles), o_fill = take(4 * n1),
            o_stats = take(8 * 4 * (size_t)egs::kStatSlots), o_statsum = take(8 * 8),
            o_sigma = take(4 * n1), o_h = take(8 * n1), o_hist = take(4 * 256), o_bfcnt = take(4 * 64);
+    // phased dense rounds: slice offsets, slice-major edges, partial keys, scan tiles
+    const bool phased = n_cap >= egs::kPhaseMinN;
+    const size_t pn1 = (size_t)egs::kPhases * n_cap + 1;
+    const size_t o_sptr = phased ? take(4 * pn1) : 0, o_sedges = phased ? take(8 * (size_t)m_cap) : 0,
+                 o_acc = phased ? take(4 * n1) : 0, o_stile = phased ? take(4 * (pn1 / 4096 + 2)) : 0;
     s->block_bytes = off;
     if (hipMalloc(&s->block, off) != hipSuccess) {
         egs_solver_destroy(s);
@@ -250,10 +260,17 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
     s->d_h = (long long *)(b + o_h);
     s->d_hist = (int *)(b + o_hist);
     s->d_bfcnt = (int *)(b + o_bfcnt);
+    if (phased) {
+        s->d_sptr = (int *)(b + o_sptr);
+        s->d_sedges = (int2 *)(b + o_sedges);
+        s->d_acc = (int *)(b + o_acc);
+        s->d_stile = (int *)(b + o_stile);
+    }
     if (hipHostMalloc((void **)&s->h_probe, 8192, hipHostMallocDefault) != hipSuccess) {
         egs_solver_destroy(s);
         return fail(EGS_E_NOMEM, "hipHostMalloc failed");
     }
+    s->h_probe[kPinOne] = 1;
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&s->ev_probe[0]) != hipSuccess || hipEventCreate(&s->ev_probe[1]) != hipSuccess ||
         hipEventCreate(&s->ev_a) != hipSuccess || hipEventCreate(&s->ev_b) != hipSuccess) {
@@ -370,7 +387,27 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     s->cert_every = o->certify_every == 0 ? kCertEvery : (o->certify_every < 0 ? 0 : o->certify_every);
     s->cert_max_passes = o->cert_max_passes > 0 ? o->cert_max_passes : 1024;
     g.rise = s->cert_every > 0 ? s->d_sigma : nullptr;    // rise stamps only when proofs run
+    // phased dense rounds where the value array outgrows the L2 (and the buffers exist)
+    const bool want = o->slices == 0 ? g.n >= egs::kPhaseMinN : o->slices > 1;
+    g.phases = (want && s->d_sptr) ? egs::kPhases : 1;
+    g.slice_len = (g.n + egs::kPhases - 1) / egs::kPhases;
+    g.sptr = s->d_sptr;
+    g.sedges = s->d_sedges;
+    g.acc = s->d_acc;
+    int pg = 1;
+    while (pg < 8 && pg * egs::kPhases < avg) pg <<= 1;
+    sh.phase_group = pg;
+    sh.phase_grid = egs::grid_for((int64_t)g.n * pg, 256, 4096);
     return EGS_OK;
+}
+
+// Transpose on the solver stream, then open the sparse round modes (ring[12] = 1).
+static hipError_t build_csc(egs_solver *s) {
+    hipError_t e = egs::launch_transpose(s->g, s->d_tile, s->d_fill, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_ring + 12, s->h_probe + kPinOne, sizeof(int), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) s->csc_built = true;
+    return e;
 }
 
 int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
@@ -381,8 +418,15 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     HIP_TRY(hipEventRecord(s->ev_a, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_stats, 0, sizeof(unsigned long long) * 4 * egs::kStatSlots, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_rounds, 0, sizeof(int), s->stream));
-    HIP_TRY(egs::launch_transpose(s->g, s->d_tile, s->d_fill, s->stream));
+    // The transpose only feeds the sparse round modes, and many solves never need them
+    // (config-2-like games sweep densely until a proof ends them).  Rounds stay dense while
+    // it is missing; a round that would have gone sparse raises ring[13], and the host
+    // builds the transpose at its next probe and opens sparse rounds (ring[12]).
+    s->csc_built = false;
+    HIP_TRY(hipMemsetAsync(s->d_ring + 12, 0, 2 * sizeof(int), s->stream));
+    if (s->g.dense_min > s->g.n) HIP_TRY(build_csc(s));     // dense sweeps switched off
     HIP_TRY(egs::launch_classify(s->g, s->d_ring + 9, s->stream));
+    HIP_TRY(egs::launch_slices(s->g, s->d_stile, s->stream));
     HIP_TRY(egs::launch_init(s->g, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_ring + 11, 0, sizeof(int), s->stream));
     HIP_TRY(hipEventRecord(s->ev_b, s->stream));
@@ -471,10 +515,15 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
         HIP_TRY(hipMemcpyAsync(s->h_probe + kPinProbe + slot * 128,
                                s->d_lcnt + (s->next_round % 3) * egs::kListCounters,
                                sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
+        if (!s->csc_built)
+            HIP_TRY(hipMemcpyAsync(s->h_probe + kPinProbe + slot * 128 + kProbeWant, s->d_ring + 13, sizeof(int),
+                                   hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipEventRecord(s->ev_probe[slot], s->stream));
         if (pending >= 0) {
             HIP_TRY(hipEventSynchronize(s->ev_probe[pending]));
             if (list_total(s, s->h_probe + kPinProbe + pending * 128) == 0) done = true;
+            // a round of that batch wanted to go sparse: build the transpose now
+            if (!done && !s->csc_built && s->h_probe[kPinProbe + pending * 128 + kProbeWant]) HIP_TRY(build_csc(s));
         }
         pending = slot;
         if (s->stop_flag && *s->stop_flag) break;
@@ -511,6 +560,7 @@ void *egs_solver_stream(egs_solver *s) { return s ? (void *)s->stream : nullptr;
 int egs_solver_read_csc(egs_solver *s, int32_t *cptr, int32_t *cpred) {
     if (!s || !s->prepared) return fail(EGS_E_STATE, "egs_solver_read_csc: not prepared");
     HIP_TRY(hipSetDevice(s->device));
+    if (!s->csc_built) HIP_TRY(build_csc(s));
     HIP_TRY(hipMemcpyAsync(cptr, s->d_cptr, sizeof(int) * ((size_t)s->g.n + 1), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipMemcpyAsync(cpred, s->d_cpred, sizeof(int) * (size_t)s->g.m, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));

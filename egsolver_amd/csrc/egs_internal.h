@@ -34,7 +34,9 @@ struct DevGame {
     int *mode;            // 3 slots: mode of round r at r % 3
     int *dbig, *dhuge;    // per-round deferral lists (sparse modes)
     int *ring;            // [3..5] dbig counters, [6..8] dhuge counters, [9] nsbig, [10] nshuge,
-                          // [11] a round that must sweep all vertices (after a TOP proof)
+                          // [11] a round that must sweep all vertices (after a TOP proof),
+                          // [12] 1 once the transpose exists (sparse rounds allowed),
+                          // [13] 1 once a round would have gone sparse without it
     const int *sbig;      // static: vertices with big_deg < deg <= huge_deg
     const int *shuge;     // static: vertices with deg > huge_deg
     int big_deg, huge_deg;
@@ -43,7 +45,17 @@ struct DevGame {
     int *rounds_done;     // last round that was not MODE_DONE
     int *rise;            // n: round in which each vertex last rose (TOP proof candidates);
                           // null when proofs are off
+    // Dense rounds in successor-range phases (phases > 1): phase s relaxes the edges whose
+    // successor lies in [s * slice_len, (s+1) * slice_len), so its value gathers stay in L2.
+    int phases;           // 1 = one in-place sweep (body_dense)
+    int slice_len;
+    const int *sptr;      // phases * n + 1 offsets: edges of (slice s, vertex v) start at sptr[s*n + v]
+    const int2 *sedges;   // the low-degree vertices' edges grouped by (slice, vertex)
+    int *acc;             // n partial keys carried between phases
 };
+
+constexpr int kPhases = 4;                 // successor slices of a phased dense round
+constexpr int kPhaseMinN = 1 << 20;        // below this the value array fits the L2 anyway
 
 // Launch shape of one round.
 struct LaunchShape {
@@ -51,12 +63,16 @@ struct LaunchShape {
     int grid;        // workgroups of k_round
     int use_wave, use_block;
     int wave_grid, block_grid;
+    int phase_group; // lanes per vertex in k_phase (phased dense rounds)
+    int phase_grid;
 };
 
 hipError_t launch_pack(const int *succ, const int *w, int2 *edges, int64_t m, hipStream_t st);
 hipError_t launch_transpose(const DevGame &g, int *tile_sum, int *fill, hipStream_t st);
 hipError_t launch_classify(const DevGame &g, int *counts, hipStream_t st);
 hipError_t launch_init(const DevGame &g, hipStream_t st);
+hipError_t launch_slices(const DevGame &g, int *tile_sum, hipStream_t st);
+void launch_scan_inplace(int *cnt, int n, int *tile_sum, hipStream_t st);
 hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int r, hipStream_t st);
 hipError_t launch_sum_stats(const DevGame &g, int nslots, unsigned long long *out, hipStream_t st);
 hipError_t launch_degree_stats(const int *ptr, int n, int *out, hipStream_t st);

@@ -136,11 +136,20 @@ __device__ __forceinline__ int decide_mode(const DevGame &g, int r, int total) {
     if (g.ring[11] == r) return MODE_DENSE;     // values were raised outside the rounds
     const int prev = g.mode[(r + 2) % 3];
     if (prev == MODE_DONE || total == 0) return MODE_DONE;
+    int mode;
     if (prev == MODE_DENSE) {
         const int64_t est = (int64_t)total * g.m / g.n;   // vertices the risers' predecessors cover
-        return est >= g.dense_min ? MODE_DENSE : MODE_LIFTED;
+        mode = est >= g.dense_min ? MODE_DENSE : MODE_LIFTED;
+    } else {
+        mode = total >= g.dense_min ? MODE_DENSE : MODE_QUEUE;
     }
-    return total >= g.dense_min ? MODE_DENSE : MODE_QUEUE;
+    return mode;
+}
+
+// The sparse modes walk predecessor lists: until the transpose exists (ring[12], raised by
+// the host) such a round sweeps densely instead and says so in ring[13].
+__device__ __forceinline__ int gate_mode(const DevGame &g, int mode) {
+    return (mode == MODE_QUEUE || mode == MODE_LIFTED) && g.ring[12] == 0 ? MODE_DENSE : mode;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -376,7 +385,11 @@ __global__ __launch_bounds__(256) void k_round(DevGame g, int r) {
     if (threadIdx.x == 0) scnt = 0;
     load_layout(g, in_slot, pre);
     __syncthreads();
-    if (threadIdx.x == 0) smode = decide_mode(g, r, pre[kSub + 1]);
+    if (threadIdx.x == 0) {
+        const int want = decide_mode(g, r, pre[kSub + 1]);
+        smode = gate_mode(g, want);
+        if (blockIdx.x == 0 && smode != want) g.ring[13] = 1;
+    }
     __syncthreads();
     const int mode = smode;
     if (blockIdx.x == 0) {
@@ -391,6 +404,7 @@ __global__ __launch_bounds__(256) void k_round(DevGame g, int r) {
         }
     }
     if (mode == MODE_DONE) return;
+    if (mode == MODE_DENSE && g.phases > 1) return;    // the phase kernels sweep this round
     const Stage st{sbuf, &scnt, g.list[(r + 1) & 1], g.lcnt + ((r + 1) % 3) * kListCounters,
                    (int)(blockIdx.x % kSub)};
     LiftAcc acc;
@@ -399,6 +413,104 @@ __global__ __launch_bounds__(256) void k_round(DevGame g, int r) {
     else body_lifted<G>(g, st, g.list[r & 1], pre, r, acc);
     stage_flush(g, st, 1);
     flush_stats(g, acc);
+}
+
+// One phase s of a phased dense round.  Every low-degree vertex folds the keys of its
+// edges into slice s into acc[v]; the last phase applies the lift: old values are read by
+// every phase (the round is Jacobi within itself), the last writes the risen ones, stamps
+// them and stages them for the next round like body_dense.  Vertices above big_deg have
+// no edges in the slices: the static wave/block kernels lift them after the phases.
+template <int G>
+__global__ __launch_bounds__(256) void k_phase(DevGame g, int r, int s) {
+    __shared__ int sbuf[kLdsCap];
+    __shared__ int scnt;
+    if (g.mode[r % 3] != MODE_DENSE) return;
+    const bool last = s == g.phases - 1;
+    if (threadIdx.x == 0) scnt = 0;
+    __syncthreads();
+    const Stage st{sbuf, &scnt, g.list[(r + 1) & 1], g.lcnt + ((r + 1) % 3) * kListCounters,
+                   (int)(blockIdx.x % kSub)};
+    LiftAcc acc;
+    constexpr int ITEMS = 256 / G;
+    const int gl = threadIdx.x % G;
+    const int *sp = g.sptr + (size_t)s * g.n;
+    int it = 0;
+    for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < g.n; t0 += (int64_t)gridDim.x * ITEMS, it++) {
+        const int v = (int)(t0 + threadIdx.x / G);
+        const bool valid = v < g.n;
+        int b = 0, e = 0;
+        if (valid) {
+            b = sp[v];
+            e = sp[v + 1];
+        }
+        const bool owner1 = v >= g.n0;
+        int key = EGS_TOP;
+        for (int k = b + gl; k < e; k += G) {
+            const int2 ew = g.sedges[k];
+            key = min(key, egs_key(egs_ominus(g.val[ew.x], ew.y, g.mg), owner1));
+        }
+        key = egs_group_min<G>(key);
+        bool lifted = false;
+        if (valid && gl == 0) {
+            acc.edges += (uint32_t)(e - b);
+            if (!last) {
+                if (s == 0) g.acc[v] = key;
+                else if (key != EGS_TOP && key < g.acc[v]) g.acc[v] = key;
+            } else if (g.ptr[v + 1] - g.ptr[v] <= g.big_deg) {
+                const int a = g.phases > 1 ? min(g.acc[v], key) : key;
+                const int t = egs_unkey(a, owner1);
+                const int old = g.val[v];
+                acc.processed++;
+                if (old < t) {
+                    g.val[v] = t;
+                    if (g.rise) g.rise[v] = r;
+                    acc.lifted++;
+                    lifted = true;
+                }
+            }
+        }
+        if (last) {
+            stage_push(g, st, lifted, v);
+            if ((it & 7) == 7) stage_flush(g, st, kLdsCap / 2);
+        }
+    }
+    if (last) stage_flush(g, st, 1);
+    flush_stats(g, acc);
+}
+
+// Slice layout of the low-degree vertices' edges: counts per (slice, vertex) into sptr,
+// an in-place exclusive scan, then each vertex places its edges (no atomics).
+__global__ __launch_bounds__(256) void k_slice_count(DevGame g, int *__restrict__ cnt) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
+        int c[kPhases] = {};
+        const int s = g.ptr[v], e = g.ptr[v + 1];
+        if (e - s <= g.big_deg) {
+            for (int k = s; k < e; k++) {
+                const int sl = g.edges[k].x / g.slice_len;
+#pragma unroll
+                for (int q = 0; q < kPhases; q++) c[q] += sl == q;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kPhases; q++) cnt[(size_t)q * g.n + v] = c[q];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_slice_scatter(DevGame g, int2 *__restrict__ sedges) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
+        const int s = g.ptr[v], e = g.ptr[v + 1];
+        if (e - s > g.big_deg) continue;
+        int pos[kPhases];
+#pragma unroll
+        for (int q = 0; q < kPhases; q++) pos[q] = g.sptr[(size_t)q * g.n + v];
+        for (int k = s; k < e; k++) {
+            const int2 ew = g.edges[k];
+            const int sl = ew.x / g.slice_len;
+#pragma unroll
+            for (int q = 0; q < kPhases; q++)
+                if (sl == q) sedges[pos[q]++] = ew;
+        }
+    }
 }
 
 // One wavefront per vertex: the static big list in dense rounds, the deferred list in
@@ -510,8 +622,29 @@ hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int r, hipStrea
     case 32: hipLaunchKernelGGL(k_round<32>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
     default: hipLaunchKernelGGL(k_round<64>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
     }
+    if (g.phases > 1) {
+        for (int q = 0; q < g.phases; q++) {
+            switch (sh.phase_group) {
+            case 1: hipLaunchKernelGGL(k_phase<1>, dim3(sh.phase_grid), dim3(256), 0, st, g, r, q); break;
+            case 2: hipLaunchKernelGGL(k_phase<2>, dim3(sh.phase_grid), dim3(256), 0, st, g, r, q); break;
+            case 4: hipLaunchKernelGGL(k_phase<4>, dim3(sh.phase_grid), dim3(256), 0, st, g, r, q); break;
+            default: hipLaunchKernelGGL(k_phase<8>, dim3(sh.phase_grid), dim3(256), 0, st, g, r, q); break;
+            }
+        }
+    }
     if (sh.use_wave) hipLaunchKernelGGL(k_lift_wave, dim3(sh.wave_grid), dim3(256), 0, st, g, r);
     if (sh.use_block) hipLaunchKernelGGL(k_lift_block, dim3(sh.block_grid), dim3(256), 0, st, g, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_slices(const DevGame &g, int *tile_sum, hipStream_t st) {
+    if (g.phases <= 1) return hipSuccess;
+    int *cnt = const_cast<int *>(g.sptr);
+    const int64_t total = (int64_t)g.phases * g.n;
+    hipLaunchKernelGGL(k_slice_count, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, st, g, cnt);
+    launch_scan_inplace(cnt, (int)total, tile_sum, st);
+    hipLaunchKernelGGL(k_slice_scatter, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, st, g,
+                       const_cast<int2 *>(g.sedges));
     return hipGetLastError();
 }
 

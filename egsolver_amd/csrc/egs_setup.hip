@@ -238,6 +238,15 @@ hipError_t launch_pack(const int *succ, const int *w, int2 *edges, int64_t m, hi
     return hipGetLastError();
 }
 
+// Exclusive scan of cnt[0..n) into cnt itself, with cnt[n] = the total (k_scan_apply
+// reads each tile into LDS before writing it back).
+void launch_scan_inplace(int *cnt, int n, int *tile_sum, hipStream_t st) {
+    const int tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(SCAN_THREADS), 0, st, cnt, n, tile_sum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_THREADS), 0, st, tile_sum, tiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(SCAN_THREADS), 0, st, cnt, n, tile_sum, cnt);
+}
+
 hipError_t launch_transpose(const DevGame &g, int *tile_sum, int *fill, hipStream_t st) {
     const int n = g.n;
     const int64_t m = g.m;

@@ -16,10 +16,10 @@ from .games import Game
 
 
 def options(group=0, big_deg=0, huge_deg=0, grid=0, batch_rounds=0, max_phases=-1, stop_flag=None, dense_pct=0,
-            certify_every=0, cert_max_passes=0):
+            certify_every=0, cert_max_passes=0, slices=0):
     o = N.EgsOptions(group=group, big_deg=big_deg, huge_deg=huge_deg, grid=grid,
                      batch_rounds=batch_rounds, dense_pct=dense_pct, certify_every=certify_every,
-                     cert_max_passes=cert_max_passes, max_phases=max_phases, stop_flag=None)
+                     cert_max_passes=cert_max_passes, max_phases=max_phases, stop_flag=None, slices=slices)
     if stop_flag is not None:
         o.stop_flag = ctypes.cast(ctypes.pointer(stop_flag), ctypes.c_void_p)
     return o

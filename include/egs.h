@@ -57,6 +57,10 @@ typedef struct egs_options {
     const volatile uint32_t *stop_flag;  /* polled between batches; non-zero stops the
                                run early with partial values (the reference's
                                timeout_expired, src/sig_handling.c:57-64) */
+    int32_t slices;         /* dense rounds in successor-range phases (their value gathers
+                               stay in L2): 0 = auto (4 phases when n >= 2^20 and the solver
+                               was created with n_cap >= 2^20), 1 = one sweep, > 1 = phases */
+    int32_t reserved;
 } egs_options;
 
 typedef struct egs_stats {

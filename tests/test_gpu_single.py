@@ -137,3 +137,47 @@ def test_partial_run_is_under_approximation(oracle):
         conv, st = s.iterate()
         assert conv
         assert np.array_equal(s.values(), ref)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_phased_dense_rounds_vs_oracle(oracle, seed):
+    """Dense rounds in successor-range phases (forced on small games through a solver sized
+    for them), with proofs on and off and with every round dense: exact."""
+    import egsolver_amd as E
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(300, 6000))
+    kind = seed % 3
+    if kind == 2:
+        p = E.GenParams(n=n, seed=700 + seed, deg_kind=2, deg_lo=1, deg_hi=min(n - 1, 3000), alpha=1.8, wmax=50)
+    else:
+        p = E.GenParams(n=n, seed=700 + seed, deg_kind=kind, deg_lo=2, deg_hi=12, wmax=int(rng.integers(5, 200)))
+    g = E.generate(p)
+    _say(f"phased seed {seed} n {g.n} m {g.m}")
+    ref = lfp(oracle, to_oracle_game(g))
+    with E.Solver(1 << 20, max(g.m, 1 << 22)) as s:
+        s.load(g)
+        for opt in (E.options(slices=4), E.options(slices=4, certify_every=-1), E.options(slices=4, dense_pct=1),
+                    E.options(slices=4, group=1, big_deg=4)):
+            s.prepare(opt)
+            conv, st = s.iterate()
+            assert conv
+            assert np.array_equal(s.values(), ref), opt
+            assert s.violations() == 0
+
+
+def test_phased_equals_single_sweep_full_size():
+    """Config 2 at full size: phased and single-sweep dense rounds end on the same values
+    (no oracle finishes this size; the device fixpoint check covers it)."""
+    import egsolver_amd as E
+    g = E.config_game(2)
+    _say(f"config 2 full size, phased vs single sweep, n {g.n} m {g.m}")
+    with E.Solver(g.n, g.m) as s:
+        s.load(g)
+        out = []
+        for sl in (1, 4):
+            s.prepare(E.options(slices=sl))
+            conv, st = s.iterate()
+            assert conv
+            assert s.violations() == 0
+            out.append(s.values())
+        assert np.array_equal(out[0], out[1])
