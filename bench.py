@@ -114,7 +114,7 @@ def single_game(E, g, args):
              "achieved_gbps": achieved}
     log(f"plain rounds: {per_launch_ms:.3f} ms/round, {plain['edges_per_s']:.3e} edges/s, {achieved:.0f} GB/s")
     # full solves to fixpoint (proofs on)
-    opt = E.options(group=shape["group"], grid=shape["grid"])
+    opt = E.options(group=shape["group"], grid=shape["grid"], certify_every=args.certify_every)
     for _ in range(args.warmup_solves):
         s.prepare(opt)
         s.iterate()
@@ -279,6 +279,7 @@ def main():
     ap.add_argument("--group", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--tune", action="store_true", help="autotune launch shapes before timing")
+    ap.add_argument("--certify-every", type=int, default=0, help="rounds between proof attempts (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-batch", action="store_true")
@@ -328,7 +329,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": plain["achieved_gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": plain["achieved_gbps"] / HBM_PEAK_GBPS, "traffic": traffic,
-            "kernel": "k_round (dense sweep)",
+            "kernel": "dense round: k_round + k_phase x 4 (successor-range phases) when n >= 2^20, else k_round",
             "algorithmic_bytes_per_launch": plain["algorithmic_bytes_per_round"],
             "avg_launch_ms": plain["device_ms_per_round"],
             "copy_kernel_gbps": copy_gbps,

@@ -69,7 +69,7 @@ constexpr int kPinOne = 1281;     // the constant 1 (raises ring[12])
 constexpr int kProbeWant = 100;   // offset in a probe slot: ring[13] (a round wanted to go sparse)
 constexpr int kCertFirstCap = 64;     // longest-path passes per candidate at the first attempt
 constexpr int kCertCandidates = 8;    // gap thresholds tried per attempt (plus the last-round rise set)
-constexpr int kCertEvery = 16;        // default rounds before the first proof attempt
+constexpr int kCertEvery = 12;        // default rounds before the first proof attempt
 
 int pow2ceil(int x) {
     int p = 1;
@@ -95,6 +95,7 @@ struct egs_solver {
     int *d_sigma = nullptr, *d_hist = nullptr, *d_bfcnt = nullptr;   // d_sigma: rise stamps
     int *d_sptr = nullptr, *d_acc = nullptr, *d_stile = nullptr;    // phased dense rounds (large n_cap)
     int2 *d_sedges = nullptr;
+    unsigned long long *d_bigmask = nullptr;
     long long *d_h = nullptr;
     int wmax = 0;
     int cert_every = 32, cert_gap = 32, cert_max_passes = 1024;
@@ -228,7 +229,8 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
     const bool phased = n_cap >= egs::kPhaseMinN;
     const size_t pn1 = (size_t)egs::kPhases * n_cap + 1;
     const size_t o_sptr = phased ? take(4 * pn1) : 0, o_sedges = phased ? take(8 * (size_t)m_cap) : 0,
-                 o_acc = phased ? take(4 * n1) : 0, o_stile = phased ? take(4 * (pn1 / 4096 + 2)) : 0;
+                 o_acc = phased ? take(4 * n1) : 0, o_stile = phased ? take(4 * (pn1 / 4096 + 2)) : 0,
+                 o_bigmask = phased ? take(8 * ((size_t)n_cap / 64 + 2)) : 0;
     s->block_bytes = off;
     if (hipMalloc(&s->block, off) != hipSuccess) {
         egs_solver_destroy(s);
@@ -265,6 +267,7 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
         s->d_sedges = (int2 *)(b + o_sedges);
         s->d_acc = (int *)(b + o_acc);
         s->d_stile = (int *)(b + o_stile);
+        s->d_bigmask = (unsigned long long *)(b + o_bigmask);
     }
     if (hipHostMalloc((void **)&s->h_probe, 8192, hipHostMallocDefault) != hipSuccess) {
         egs_solver_destroy(s);
@@ -393,7 +396,9 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     g.slice_len = (g.n + egs::kPhases - 1) / egs::kPhases;
     g.sptr = s->d_sptr;
     g.sedges = s->d_sedges;
+    g.sshift = o->slices == 1 ? 0 : egs::pack_shift(g.n, s->wmax);   // 32-bit slice edges when they fit
     g.acc = s->d_acc;
+    g.bigmask = s->d_bigmask;
     int pg = 1;
     while (pg < 8 && pg * egs::kPhases < avg) pg <<= 1;
     sh.phase_group = pg;

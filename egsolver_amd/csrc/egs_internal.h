@@ -50,7 +50,10 @@ struct DevGame {
     int phases;           // 1 = one in-place sweep (body_dense)
     int slice_len;
     const int *sptr;      // phases * n + 1 offsets: edges of (slice s, vertex v) start at sptr[s*n + v]
-    const int2 *sedges;   // the low-degree vertices' edges grouped by (slice, vertex)
+    const void *sedges;   // the low-degree vertices' edges grouped by (slice, vertex): uint32
+                          // (w << sshift) | succ when sshift > 0, else int2 {succ, w}
+    int sshift;
+    const unsigned long long *bigmask;   // bit v: vertex v is above big_deg (lifted by the wave/block kernels)
     int *acc;             // n partial keys carried between phases
 };
 
@@ -72,6 +75,7 @@ hipError_t launch_transpose(const DevGame &g, int *tile_sum, int *fill, hipStrea
 hipError_t launch_classify(const DevGame &g, int *counts, hipStream_t st);
 hipError_t launch_init(const DevGame &g, hipStream_t st);
 hipError_t launch_slices(const DevGame &g, int *tile_sum, hipStream_t st);
+int pack_shift(int n, int wmax);
 void launch_scan_inplace(int *cnt, int n, int *tile_sum, hipStream_t st);
 hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int r, hipStream_t st);
 hipError_t launch_sum_stats(const DevGame &g, int nslots, unsigned long long *out, hipStream_t st);

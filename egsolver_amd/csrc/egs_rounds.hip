@@ -420,7 +420,20 @@ __global__ __launch_bounds__(256) void k_round(DevGame g, int r) {
 // every phase (the round is Jacobi within itself), the last writes the risen ones, stamps
 // them and stages them for the next round like body_dense.  Vertices above big_deg have
 // no edges in the slices: the static wave/block kernels lift them after the phases.
-template <int G>
+template <bool PACK>
+__device__ __forceinline__ void slice_edge(const DevGame &g, int k, int &succ, int &w) {
+    if (PACK) {
+        const uint32_t e = ((const uint32_t *)g.sedges)[k];
+        succ = (int)(e & ((1u << g.sshift) - 1u));
+        w = ((int)e) >> g.sshift;
+    } else {
+        const int2 e = ((const int2 *)g.sedges)[k];
+        succ = e.x;
+        w = e.y;
+    }
+}
+
+template <int G, bool PACK>
 __global__ __launch_bounds__(256) void k_phase(DevGame g, int r, int s) {
     __shared__ int sbuf[kLdsCap];
     __shared__ int scnt;
@@ -446,8 +459,9 @@ __global__ __launch_bounds__(256) void k_phase(DevGame g, int r, int s) {
         const bool owner1 = v >= g.n0;
         int key = EGS_TOP;
         for (int k = b + gl; k < e; k += G) {
-            const int2 ew = g.sedges[k];
-            key = min(key, egs_key(egs_ominus(g.val[ew.x], ew.y, g.mg), owner1));
+            int u, w;
+            slice_edge<PACK>(g, k, u, w);
+            key = min(key, egs_key(egs_ominus(g.val[u], w, g.mg), owner1));
         }
         key = egs_group_min<G>(key);
         bool lifted = false;
@@ -456,7 +470,7 @@ __global__ __launch_bounds__(256) void k_phase(DevGame g, int r, int s) {
             if (!last) {
                 if (s == 0) g.acc[v] = key;
                 else if (key != EGS_TOP && key < g.acc[v]) g.acc[v] = key;
-            } else if (g.ptr[v + 1] - g.ptr[v] <= g.big_deg) {
+            } else if (!((g.bigmask[v >> 6] >> (v & 63)) & 1ull)) {
                 const int a = g.phases > 1 ? min(g.acc[v], key) : key;
                 const int t = egs_unkey(a, owner1);
                 const int old = g.val[v];
@@ -480,23 +494,32 @@ __global__ __launch_bounds__(256) void k_phase(DevGame g, int r, int s) {
 
 // Slice layout of the low-degree vertices' edges: counts per (slice, vertex) into sptr,
 // an in-place exclusive scan, then each vertex places its edges (no atomics).
-__global__ __launch_bounds__(256) void k_slice_count(DevGame g, int *__restrict__ cnt) {
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
-        int c[kPhases] = {};
-        const int s = g.ptr[v], e = g.ptr[v + 1];
-        if (e - s <= g.big_deg) {
-            for (int k = s; k < e; k++) {
-                const int sl = g.edges[k].x / g.slice_len;
+// Also marks the vertices above big_deg in bigmask (one bit per vertex, from a wave ballot).
+__global__ __launch_bounds__(256) void k_slice_count(DevGame g, int *__restrict__ cnt,
+                                                     unsigned long long *__restrict__ bigmask) {
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < g.n; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = base + threadIdx.x;
+        bool big = false;
+        if (v < g.n) {
+            int c[kPhases] = {};
+            const int s = g.ptr[v], e = g.ptr[v + 1];
+            big = e - s > g.big_deg;
+            if (!big) {
+                for (int k = s; k < e; k++) {
+                    const int sl = g.edges[k].x / g.slice_len;
 #pragma unroll
-                for (int q = 0; q < kPhases; q++) c[q] += sl == q;
+                    for (int q = 0; q < kPhases; q++) c[q] += sl == q;
+                }
             }
-        }
 #pragma unroll
-        for (int q = 0; q < kPhases; q++) cnt[(size_t)q * g.n + v] = c[q];
+            for (int q = 0; q < kPhases; q++) cnt[(size_t)q * g.n + v] = c[q];
+        }
+        const unsigned long long word = __ballot(big);
+        if ((threadIdx.x & 63) == 0 && v < g.n) bigmask[v >> 6] = word;
     }
 }
 
-__global__ __launch_bounds__(256) void k_slice_scatter(DevGame g, int2 *__restrict__ sedges) {
+__global__ __launch_bounds__(256) void k_slice_scatter(DevGame g, void *__restrict__ sedges) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
         const int s = g.ptr[v], e = g.ptr[v + 1];
         if (e - s > g.big_deg) continue;
@@ -507,8 +530,11 @@ __global__ __launch_bounds__(256) void k_slice_scatter(DevGame g, int2 *__restri
             const int2 ew = g.edges[k];
             const int sl = ew.x / g.slice_len;
 #pragma unroll
-            for (int q = 0; q < kPhases; q++)
-                if (sl == q) sedges[pos[q]++] = ew;
+            for (int q = 0; q < kPhases; q++) {
+                if (sl != q) continue;
+                if (g.sshift) ((uint32_t *)sedges)[pos[q]++] = ((uint32_t)ew.y << g.sshift) | (uint32_t)ew.x;
+                else ((int2 *)sedges)[pos[q]++] = ew;
+            }
         }
     }
 }
@@ -624,11 +650,21 @@ hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int r, hipStrea
     }
     if (g.phases > 1) {
         for (int q = 0; q < g.phases; q++) {
-            switch (sh.phase_group) {
-            case 1: hipLaunchKernelGGL(k_phase<1>, dim3(sh.phase_grid), dim3(256), 0, st, g, r, q); break;
-            case 2: hipLaunchKernelGGL(k_phase<2>, dim3(sh.phase_grid), dim3(256), 0, st, g, r, q); break;
-            case 4: hipLaunchKernelGGL(k_phase<4>, dim3(sh.phase_grid), dim3(256), 0, st, g, r, q); break;
-            default: hipLaunchKernelGGL(k_phase<8>, dim3(sh.phase_grid), dim3(256), 0, st, g, r, q); break;
+            const dim3 gr(sh.phase_grid), bl(256);
+            if (g.sshift) {
+                switch (sh.phase_group) {
+                case 1: hipLaunchKernelGGL((k_phase<1, true>), gr, bl, 0, st, g, r, q); break;
+                case 2: hipLaunchKernelGGL((k_phase<2, true>), gr, bl, 0, st, g, r, q); break;
+                case 4: hipLaunchKernelGGL((k_phase<4, true>), gr, bl, 0, st, g, r, q); break;
+                default: hipLaunchKernelGGL((k_phase<8, true>), gr, bl, 0, st, g, r, q); break;
+                }
+            } else {
+                switch (sh.phase_group) {
+                case 1: hipLaunchKernelGGL((k_phase<1, false>), gr, bl, 0, st, g, r, q); break;
+                case 2: hipLaunchKernelGGL((k_phase<2, false>), gr, bl, 0, st, g, r, q); break;
+                case 4: hipLaunchKernelGGL((k_phase<4, false>), gr, bl, 0, st, g, r, q); break;
+                default: hipLaunchKernelGGL((k_phase<8, false>), gr, bl, 0, st, g, r, q); break;
+                }
             }
         }
     }
@@ -641,10 +677,11 @@ hipError_t launch_slices(const DevGame &g, int *tile_sum, hipStream_t st) {
     if (g.phases <= 1) return hipSuccess;
     int *cnt = const_cast<int *>(g.sptr);
     const int64_t total = (int64_t)g.phases * g.n;
-    hipLaunchKernelGGL(k_slice_count, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, st, g, cnt);
+    hipLaunchKernelGGL(k_slice_count, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, st, g, cnt,
+                       const_cast<unsigned long long *>(g.bigmask));
     launch_scan_inplace(cnt, (int)total, tile_sum, st);
     hipLaunchKernelGGL(k_slice_scatter, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, st, g,
-                       const_cast<int2 *>(g.sedges));
+                       const_cast<void *>(g.sedges));
     return hipGetLastError();
 }
 

@@ -181,3 +181,20 @@ def test_phased_equals_single_sweep_full_size():
             assert s.violations() == 0
             out.append(s.values())
         assert np.array_equal(out[0], out[1])
+
+
+def test_phased_unpacked_edges_equal_single_sweep():
+    """Weights too wide to share a 32-bit word with a 17-bit successor id keep the slice
+    edges as int2; phased and single-sweep rounds end on the same fixpoint."""
+    import egsolver_amd as E
+    g = E.generate(E.GenParams(n=1 << 17, seed=91, deg_kind=1, deg_lo=2, deg_hi=6, wmax=20000))
+    _say(f"phased unpacked n {g.n} m {g.m} mg {g.mg}")
+    with E.Solver(1 << 20, 1 << 22) as s:
+        s.load(g)
+        out = []
+        for sl in (1, 4):
+            s.prepare(E.options(slices=sl))
+            conv, st = s.iterate()
+            assert conv and s.violations() == 0
+            out.append(s.values())
+        assert np.array_equal(out[0], out[1])
