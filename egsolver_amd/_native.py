@@ -132,6 +132,8 @@ EGS_SIGNATURES = {
     "egs_solver_destroy": (ctypes.c_int, [_vp]),
     "egs_solver_load": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
                                        ctypes.c_int32, _vp, _vp, _vp]),
+    "egs_solver_load_packed": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                                              _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "egs_solver_prepare": (ctypes.c_int, [_vp, ctypes.POINTER(EgsOptions)]),
     "egs_solver_iterate": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.POINTER(EgsStats)]),
     "egs_solver_stats": (ctypes.c_int, [_vp, ctypes.POINTER(EgsStats)]),

@@ -284,6 +284,8 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
     return EGS_OK;
 }
 
+static void bind_game(egs_solver *s, int32_t n, int32_t n0, int64_t m, int32_t mg);
+
 int egs_solver_load(egs_solver *s, int mem, int32_t n, int32_t n0, int64_t m, int32_t mg,
                     const int32_t *ptr, const int32_t *succ, const int32_t *w) {
     if (!s) return fail(EGS_E_ARG, "null solver");
@@ -295,10 +297,15 @@ int egs_solver_load(egs_solver *s, int mem, int32_t n, int32_t n0, int64_t m, in
                     (long long)m, s->n_cap, (long long)s->m_cap);
     HIP_TRY(hipSetDevice(s->device));
     if (mem == EGS_MEM_HOST) {
-        // validate the CSR on the host before anything reaches the device
+        // validate the CSR on the host (fused with the int2 packing) before anything reaches
+        // the device
+        int2 *packed = (int2 *)malloc(sizeof(int2) * (size_t)m);
+        if (!packed) return fail(EGS_E_NOMEM, "host staging allocation failed");
         int32_t wm = 0, mx = 0;
         int64_t bad = -1;
-        switch (egs_pk_check(n, m, ptr, succ, w, &wm, &mx, &bad)) {
+        const int chk = egs_pk_check_pack64(n, m, ptr, succ, w, (int32_t *)packed, &wm, &mx, &bad);
+        if (chk) free(packed);
+        switch (chk) {
         case EGS_PK_BAD_OFFSETS:
             return fail(EGS_E_ARG, "CSR offsets must run from 0 to m (got %d..%d)", ptr[0], ptr[n]);
         case EGS_PK_NO_EDGE: return fail(EGS_E_ARG, "vertex %lld has no out-edge", (long long)bad);
@@ -308,9 +315,6 @@ int egs_solver_load(egs_solver *s, int mem, int32_t n, int32_t n0, int64_t m, in
         }
         s->max_outdeg = mx;
         s->wmax = wm;
-        int2 *packed = (int2 *)malloc(sizeof(int2) * (size_t)m);
-        if (!packed) return fail(EGS_E_NOMEM, "host staging allocation failed");
-        egs_pk_pack64(m, succ, w, (int32_t *)packed);
         hipError_t e1 = hipMemcpy(s->d_ptr, ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice);
         hipError_t e2 = hipMemcpy(s->d_edges, packed, sizeof(int2) * (size_t)m, hipMemcpyHostToDevice);
         free(packed);
@@ -334,6 +338,32 @@ int egs_solver_load(egs_solver *s, int mem, int32_t n, int32_t n0, int64_t m, in
     } else {
         return fail(EGS_E_ARG, "mem must be EGS_MEM_HOST or EGS_MEM_DEVICE");
     }
+    bind_game(s, n, n0, m, mg);
+    return EGS_OK;
+}
+
+int egs_solver_load_packed(egs_solver *s, int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *d_ptr,
+                           const uint32_t *d_edges, int32_t shift, int32_t wmax, int32_t max_outdeg) {
+    if (!s) return fail(EGS_E_ARG, "null solver");
+    if (n < 1 || n0 < 0 || n0 > n || m < 1 || mg < 0 || !d_ptr || !d_edges || shift < 1 || shift > 30 || wmax < 0 ||
+        max_outdeg < 1)
+        return fail(EGS_E_ARG, "egs_solver_load_packed: bad game (n=%d n0=%d m=%lld mg=%d shift=%d)", n, n0,
+                    (long long)m, mg, shift);
+    if (n > s->n_cap || m > s->m_cap)
+        return fail(EGS_E_CAPACITY, "game (n=%d, m=%lld) exceeds solver capacity (%d, %lld)", n,
+                    (long long)m, s->n_cap, (long long)s->m_cap);
+    HIP_TRY(hipSetDevice(s->device));
+    // device inputs are trusted to be a checked CSR of this shape (the caller owns them)
+    HIP_TRY(hipMemcpyAsync(s->d_ptr, d_ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(egs::launch_unpack(d_edges, shift, s->d_edges, m, s->stream));
+    s->max_outdeg = max_outdeg;
+    s->wmax = wmax;
+    bind_game(s, n, n0, m, mg);
+    return EGS_OK;
+}
+
+// Point the device-game view at the solver's arrays for a freshly loaded game.
+static void bind_game(egs_solver *s, int32_t n, int32_t n0, int64_t m, int32_t mg) {
     egs::DevGame &g = s->g;
     g.n = n;
     g.n0 = n0;
@@ -359,7 +389,6 @@ int egs_solver_load(egs_solver *s, int mem, int32_t n, int32_t n0, int64_t m, in
     g.rounds_done = s->d_rounds;
     s->loaded = true;
     s->prepared = false;
-    return EGS_OK;
 }
 
 static int pick_shape(egs_solver *s, const egs_options *o) {

@@ -136,7 +136,8 @@ __device__ __forceinline__ bool game_in_d(const GameDesc &b, const int *val, int
     return x != EGS_TOP && (B >= 0 ? x > B : b.rise[v] == last);
 }
 
-__device__ int game_certify(const GameDesc &b, int *val, int last, int cap, int max_cands, int64_t &passes_done,
+__device__ int game_certify(const GameDesc &b, int *val, long long *hp, int last, int cap, int max_cands,
+                            int64_t &passes_done,
                             bool &cap_hit) {
     __shared__ int win[64], flo[64];
     __shared__ unsigned long long s_cand;
@@ -193,10 +194,10 @@ __device__ int game_certify(const GameDesc &b, int *val, int last, int cap, int 
         int members = 0;
         for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
             // h >= 0 marks D for the passes; sentinels elsewhere (one 8-byte read per edge)
-            if (val[v] == EGS_TOP) { b.h[v] = kGameHTop; continue; }
-            if (!game_in_d(b, val, v, B, last)) { b.h[v] = kGameHOut; continue; }
+            if (val[v] == EGS_TOP) { hp[v] = kGameHTop; continue; }
+            if (!game_in_d(b, val, v, B, last)) { hp[v] = kGameHOut; continue; }
             members++;
-            b.h[v] = scale * ((long long)b.mg - val[v]);
+            hp[v] = scale * ((long long)b.mg - val[v]);
             const int s = b.ptr[v], e = b.ptr[v + 1];
             bool any = false, all = true;
             for (int k = s; k < e; k++) {
@@ -214,7 +215,7 @@ __device__ int game_certify(const GameDesc &b, int *val, int last, int cap, int 
         for (int p = 0; p < cap; p++) {
             bool ch = false, over = false;
             for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
-                const long long hv = b.h[v];
+                const long long hv = hp[v];
                 if (hv < 0) continue;
                 long long best = hv;
                 const int s = b.ptr[v], e = b.ptr[v + 1];
@@ -223,7 +224,7 @@ __device__ int game_certify(const GameDesc &b, int *val, int last, int cap, int 
                     for (int k = s; k < e; k++) {
                         int u, w;
                         game_edge(b, k, u, w);
-                        const long long hu = b.h[u];
+                        const long long hu = hp[u];
                         if (hu == kGameHTop) { mn = LLONG_MIN; break; }
                         if (hu != kGameHOut) mn = min(mn, scale * w + 1 + hu);
                     }
@@ -232,12 +233,12 @@ __device__ int game_certify(const GameDesc &b, int *val, int last, int cap, int 
                     for (int k = s; k < e; k++) {
                         int u, w;
                         game_edge(b, k, u, w);
-                        const long long hu = b.h[u];
+                        const long long hu = hp[u];
                         if (hu >= 0) best = max(best, scale * w + 1 + hu);
                     }
                 }
                 if (best > hv) {
-                    b.h[v] = best;
+                    hp[v] = best;
                     ch = true;
                     over |= best > bound;
                 }
@@ -253,7 +254,7 @@ __device__ int game_certify(const GameDesc &b, int *val, int last, int cap, int 
         }
         int mine = 0;
         for (int v = threadIdx.x; v < b.n; v += blockDim.x)
-            if (b.h[v] >= 0) { val[v] = EGS_TOP; mine++; }
+            if (hp[v] >= 0) { val[v] = EGS_TOP; mine++; }
         if (mine) atomicAdd(&s_marked, mine);
         __syncthreads();
         return s_marked;
@@ -317,12 +318,13 @@ __device__ bool game_run(const GameDesc &b, int *val, GameState &sst, int64_t bu
 
 // One proof attempt at a sweep boundary; schedules the next one (spacing back to `every`
 // after a proof, doubled after a failure, pass cap doubled when a candidate ran out).
-__device__ void game_prove(const GameDesc &b, int *val, GameState &sst, const CertArgs &ca) {
+// `hp` holds the potentials: LDS after the values when both fit, else the game's array.
+__device__ void game_prove(const GameDesc &b, int *val, long long *hp, GameState &sst, const CertArgs &ca) {
     const int cap = sst.bf_cap;
     __syncthreads();
     int64_t passes = 0;
     bool cap_hit = false;
-    const int marked = game_certify(b, val, (int)sst.sweeps, cap, ca.candidates, passes, cap_hit);
+    const int marked = game_certify(b, val, hp, (int)sst.sweeps, cap, ca.candidates, passes, cap_hit);
     const bool hit = __syncthreads_or(cap_hit) != 0;
     if (threadIdx.x == 0) {
         sst.cert_passes += passes;
@@ -344,7 +346,7 @@ __device__ void game_prove(const GameDesc &b, int *val, GameState &sst, const Ce
 static __global__ __launch_bounds__(1024) void k_games(const GameDesc *__restrict__ games, const int *__restrict__ order,
                                                 int ngames, int *counter, int lds_cap, int64_t wg_budget,
                                                 int64_t max_sweeps, CertArgs ca) {
-    extern __shared__ int lds_val[];
+    extern __shared__ __align__(16) int lds_val[];
     __shared__ int s_idx;
     __shared__ GameState s_st;
     int64_t left = wg_budget;
@@ -357,6 +359,9 @@ static __global__ __launch_bounds__(1024) void k_games(const GameDesc *__restric
         const GameDesc b = games[order[idx]];
         const bool in_lds = b.n <= lds_cap;
         int *val = in_lds ? lds_val : b.values;
+        // proof potentials (8 bytes each) in LDS after the values when the game is small enough
+        const int h_at = (b.n + 1) & ~1;
+        long long *hp = in_lds && h_at + 2 * b.n <= lds_cap ? (long long *)(lds_val + h_at) : b.h;
         if (threadIdx.x == 0) {
             s_st = *b.state;
             if (!s_st.started) {
@@ -396,7 +401,7 @@ static __global__ __launch_bounds__(1024) void k_games(const GameDesc *__restric
             left -= spent;
             sweeps_left -= sweeps;
             if (!due) break;
-            game_prove(b, val, s_st, ca);
+            game_prove(b, val, hp, s_st, ca);
             if (left <= 0 || sweeps_left <= 0) break;
         }
         if (in_lds) {

@@ -71,6 +71,7 @@ struct LaunchShape {
 };
 
 hipError_t launch_pack(const int *succ, const int *w, int2 *edges, int64_t m, hipStream_t st);
+hipError_t launch_unpack(const uint32_t *packed, int shift, int2 *edges, int64_t m, hipStream_t st);
 hipError_t launch_transpose(const DevGame &g, int *tile_sum, int *fill, hipStream_t st);
 hipError_t launch_classify(const DevGame &g, int *counts, hipStream_t st);
 hipError_t launch_init(const DevGame &g, hipStream_t st);

@@ -60,6 +60,11 @@ struct Arena {
     std::map<size_t, size_t> free_;   // offset -> length
     size_t total = 0, in_use = 0;
     void init(size_t n) { free_.clear(); free_[0] = n; total = n; in_use = 0; }
+    bool fits(size_t n) const {
+        for (const auto &f : free_)
+            if (f.second >= n) return true;
+        return false;
+    }
     bool alloc(size_t n, size_t &at) {
         for (auto it = free_.begin(); it != free_.end(); ++it) {
             if (it->second >= n) {
@@ -93,6 +98,7 @@ struct Slot {
     int64_t m = 0;
     int64_t seq = 0;      // admission order
     int64_t sweeps = 0, edges = 0;   // large route: rounds and edges relaxed
+    int shift = 0, wmax = 0, maxdeg = 0;   // large route: edge packing (0 = separate arrays), max |w|, max degree
     int rc = 0;           // large route: the solve's return code
     std::string err;      // and its message
 };
@@ -102,6 +108,11 @@ constexpr int kLargeCus = 64;        // default compute units a slice leaves to 
 // Default route threshold: a workgroup per game pays off while the game's values stay in
 // LDS or close to it; from this many vertices the device-wide solver serves it sooner.
 constexpr int kLargeN = 65536;
+constexpr size_t kStagingInit = 96u << 20;   // largest pinned staging buffer made at creation
+// Large-route solvers are created for at least this many vertices and edges (one creation
+// covers games up to this size; larger ones grow the worker's solver).
+constexpr int kLargeSolverN = 1 << 19;
+constexpr int64_t kLargeSolverM = (int64_t)1 << 23;
 
 // One device-wide solver of the large route (grown to the largest game it has seen).
 struct LargeWorker {
@@ -119,10 +130,11 @@ size_t footprint(int n, int64_t m, int wmax) {
            al256(4 * (size_t)n) + al256(8 * (size_t)n);
 }
 
-// Arena bytes of a large-route game: offsets, successors, weights, values (never more than
-// the small layout, so footprint() bounds both).
-size_t footprint_large(int n, int64_t m) {
-    return al256(4 * ((size_t)n + 1)) + 2 * al256(4 * (size_t)m) + al256(4 * (size_t)n);
+// Arena bytes of a large-route game: offsets, edges (32-bit packed when `packed`, else
+// successors and weights), values; never more than the small layout, so footprint() bounds
+// both.
+size_t footprint_large(int n, int64_t m, bool packed) {
+    return al256(4 * ((size_t)n + 1)) + (packed ? 1 : 2) * al256(4 * (size_t)m) + al256(4 * (size_t)n);
 }
 }  // namespace
 
@@ -144,10 +156,10 @@ struct egs_server {
     std::vector<int> active;
     // Pinned staging ring: a submit packs into the next buffer after that buffer's previous
     // upload has finished, so consecutive submits do not wait for each other's copies.
-    static constexpr int kStaging = 3;
-    void *staging[kStaging] = {nullptr, nullptr, nullptr};
-    size_t staging_bytes[kStaging] = {0, 0, 0};
-    hipEvent_t staging_ev[kStaging] = {nullptr, nullptr, nullptr};
+    static constexpr int kStaging = 6;
+    void *staging[kStaging] = {};
+    size_t staging_bytes[kStaging] = {};
+    hipEvent_t staging_ev[kStaging] = {};
     int staging_next = 0;
     int lds_limit = 0, cus = 0;
     bool slice_running = false;
@@ -194,6 +206,21 @@ int staging_reserve(egs_server *s, size_t need, int *buf) {
 
 int solve_large(egs_server *s, LargeWorker &wk, int ticket, const egs_serve_options *o);
 
+// (Re)create a worker's solver for at least n vertices and m edges.
+int grow_solver(egs_server *s, LargeWorker &wk, int n, int64_t m) {
+    if (wk.solver) egs_solver_destroy(wk.solver);
+    wk.solver = nullptr;
+    int nc = kLargeSolverN;
+    while (nc < n) nc *= 2;
+    int64_t mc = kLargeSolverM;
+    while (mc < m) mc *= 2;
+    int rc = egs_solver_create(s->device, nc, mc, &wk.solver);
+    if (rc) return sfail(rc, "large-route solver: %s", egs_last_error());
+    wk.n = nc;
+    wk.m = mc;
+    return EGS_OK;
+}
+
 // Worker thread of the large route: until the server stops, take the oldest queued game,
 // wait for the uploads, solve it, hand it back.
 void large_worker_main(egs_server *s, int wi) {
@@ -220,10 +247,15 @@ void large_worker_main(egs_server *s, int wi) {
     }
 }
 
-void start_workers(egs_server *s) {
-    if (!s->threads.empty()) return;
+int start_workers(egs_server *s) {
+    if (!s->threads.empty()) return EGS_OK;
     s->workers.assign(s->large_workers, LargeWorker{});
+    for (auto &wk : s->workers) {
+        int rc = grow_solver(s, wk, kLargeSolverN, kLargeSolverM);
+        if (rc) return rc;
+    }
     for (int wi = 0; wi < s->large_workers; wi++) s->threads.emplace_back(large_worker_main, s, wi);
+    return EGS_OK;
 }
 
 void stop_workers(egs_server *s) {
@@ -241,23 +273,42 @@ void stop_workers(egs_server *s) {
     s->stopping = false;
 }
 
-// Large route: the raw CSR goes into the arena chunk; the game waits in large_queue.
-int submit_large(egs_server *s, int ticket, size_t at, size_t bytes, int32_t n, int32_t n0, int64_t m, int32_t mg,
-                 const int32_t *ptr, const int32_t *succ, const int32_t *w) {
+// Large route: the checked CSR goes into an arena chunk (edges packed to 32 bits when the
+// weights allow, like the small route); the game waits in large_queue.
+int submit_large(egs_server *s, int ticket, int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *ptr,
+                 const int32_t *succ, const int32_t *w) {
     const size_t ptr_b = 4 * ((size_t)n + 1), e_b = 4 * (size_t)m;
-    const size_t o_succ = al256(ptr_b), o_w = o_succ + al256(e_b), o_val = o_w + al256(e_b);
+    const size_t o_e = al256(ptr_b), o_w = o_e + al256(e_b);
+    int rc = start_workers(s);
+    if (rc) return rc;
     int sb = 0;
-    int rc = staging_reserve(s, o_val, &sb);
-    if (rc) {
-        s->alloc.release(at, bytes);
-        return rc;
-    }
+    rc = staging_reserve(s, o_w + al256(e_b), &sb);
+    if (rc) return rc;
     char *stg = (char *)s->staging[sb];
+    int32_t wm = 0, dm = 0;
+    int64_t bad = -1;
+    const int shift_n = egs::pack_shift(n, 0);
+    int chk = shift_n ? egs_pk_check_pack32(n, m, ptr, succ, w, shift_n, (uint32_t *)(stg + o_e), &wm, &dm, &bad)
+                      : egs_pk_check(n, m, ptr, succ, w, &wm, &dm, &bad);
+    switch (chk) {
+    case EGS_PK_BAD_OFFSETS: return sfail(EGS_E_ARG, "egs_server_submit: CSR offsets must run 0..m");
+    case EGS_PK_NO_EDGE: return sfail(EGS_E_ARG, "egs_server_submit: vertex %lld has no out-edge", (long long)bad);
+    case EGS_PK_BAD_SUCC:
+        return sfail(EGS_E_ARG, "egs_server_submit: successor out of range at edge %lld", (long long)bad);
+    default: break;
+    }
+    const int shift = egs::pack_shift(n, wm);
+    if (!shift) {                                  // weights too wide: successors and weights apart
+        egs_pk_copy(stg + o_e, succ, e_b);
+        egs_pk_copy(stg + o_w, w, e_b);
+    }
     egs_pk_copy(stg, ptr, ptr_b);
-    egs_pk_copy(stg + o_succ, succ, e_b);
-    egs_pk_copy(stg + o_w, w, e_b);
+    const size_t up = shift ? o_e + e_b : o_w + e_b;
+    const size_t bytes = footprint_large(n, m, shift > 0);
+    size_t at = 0;
+    if (!s->alloc.alloc(bytes, at)) return sfail(EGS_E_CAPACITY, "arena full (%zu bytes wanted)", bytes);
     char *base = (char *)s->arena + at;
-    SHIP(hipMemcpyAsync(base, stg, o_val, hipMemcpyHostToDevice, s->copy));
+    SHIP(hipMemcpyAsync(base, stg, up, hipMemcpyHostToDevice, s->copy));
     SHIP(hipEventRecord(s->staging_ev[sb], s->copy));
     SHIP(hipEventRecord(s->ev_upload, s->copy));
     Slot &sl = s->slots[ticket];
@@ -270,14 +321,17 @@ int submit_large(egs_server *s, int ticket, size_t at, size_t bytes, int32_t n, 
     sl.n0 = n0;
     sl.mg = mg;
     sl.m = m;
+    sl.shift = shift;
+    sl.wmax = wm;
+    sl.maxdeg = dm;
     sl.seq = s->admitted++;
+    s->hi_slot = std::max(s->hi_slot, ticket);
     egs::GameDesc d{};
     d.ptr = (const int *)base;
-    d.values = (int *)(base + o_val);
+    d.values = (int *)(base + (shift ? o_w : o_w + al256(e_b)));
     d.n = n;
     d.m = m;
     s->descs[ticket] = d;
-    start_workers(s);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->large_queue.push_back(ticket);
@@ -292,21 +346,15 @@ int submit_large(egs_server *s, int ticket, size_t at, size_t bytes, int32_t n, 
 int solve_large(egs_server *s, LargeWorker &wk, int ticket, const egs_serve_options *o) {
     Slot &sl = s->slots[ticket];
     if (!wk.solver || wk.n < sl.n || wk.m < sl.m) {
-        if (wk.solver) egs_solver_destroy(wk.solver);
-        wk.solver = nullptr;
-        int nc = 1024;
-        while (nc < sl.n) nc *= 2;
-        int64_t mc = 4096;
-        while (mc < sl.m) mc *= 2;
-        int rc = egs_solver_create(s->device, nc, mc, &wk.solver);
-        if (rc) return sfail(rc, "large-route solver: %s", egs_last_error());
-        wk.n = nc;
-        wk.m = mc;
+        int rc = grow_solver(s, wk, sl.n, sl.m);
+        if (rc) return rc;
     }
     const char *base = (const char *)s->arena + sl.at;
-    const size_t o_succ = al256(4 * ((size_t)sl.n + 1)), o_w = o_succ + al256(4 * (size_t)sl.m);
-    int rc = egs_solver_load(wk.solver, EGS_MEM_DEVICE, sl.n, sl.n0, sl.m, sl.mg, (const int32_t *)base,
-                             (const int32_t *)(base + o_succ), (const int32_t *)(base + o_w));
+    const size_t o_e = al256(4 * ((size_t)sl.n + 1)), o_w = o_e + al256(4 * (size_t)sl.m);
+    int rc = sl.shift ? egs_solver_load_packed(wk.solver, sl.n, sl.n0, sl.m, sl.mg, (const int32_t *)base,
+                                               (const uint32_t *)(base + o_e), sl.shift, sl.wmax, sl.maxdeg)
+                      : egs_solver_load(wk.solver, EGS_MEM_DEVICE, sl.n, sl.n0, sl.m, sl.mg, (const int32_t *)base,
+                                        (const int32_t *)(base + o_e), (const int32_t *)(base + o_w));
     egs_options eo{};
     eo.certify_every = o->certify_every;
     eo.cert_max_passes = o->cert_max_passes;
@@ -377,9 +425,15 @@ int egs_server_create(int device, int64_t arena_bytes, int32_t max_games, egs_se
         hipEventCreate(&s->ev_slice) != hipSuccess || hipEventCreate(&s->ev_a) != hipSuccess) {
         return sfail(EGS_E_HIP, "stream/event creation failed");
     }
-    for (int i = 0; i < egs_server::kStaging; i++)
+    for (int i = 0; i < egs_server::kStaging; i++) {
         if (hipEventCreateWithFlags(&s->staging_ev[i], hipEventDisableTiming) != hipSuccess)
             return sfail(EGS_E_HIP, "event creation failed");
+        // staging sized for typical requests up front, so admissions do not pay for growth
+        const size_t init = std::min(kStagingInit, std::max((size_t)4 << 20, (size_t)arena_bytes / 64));
+        if (hipHostMalloc(&s->staging[i], init, hipHostMallocDefault) != hipSuccess)
+            return sfail(EGS_E_NOMEM, "pinned staging allocation failed");
+        s->staging_bytes[i] = init;
+    }
     *out = s;
     return EGS_OK;
 }
@@ -427,10 +481,11 @@ int egs_server_set_route(egs_server *s, int32_t large_n, int32_t large_workers) 
     }
     s->large_n = large_n > 0 ? large_n : kLargeN;
     if (nw != s->large_workers) {
-        stop_workers(s);                       // restarted with the new count at the next large game
+        stop_workers(s);
         s->large_workers = nw;
     }
-    return EGS_OK;
+    SHIP(hipSetDevice(s->device));
+    return start_workers(s);                   // solvers and threads exist before requests arrive
 }
 
 // Admit a game: returns its ticket, or EGS_E_CAPACITY when the arena or the slot table is
@@ -439,39 +494,42 @@ int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t m
                       const int32_t *succ, const int32_t *w) {
     if (!s || n < 1 || n0 < 0 || n0 > n || m < 1 || mg < 0 || !ptr || !succ || !w)
         return sfail(EGS_E_ARG, "egs_server_submit: bad game");
+    int ticket = -1;
+    for (int i = 0; i < s->max_games; i++)
+        if (!s->slots[i].used) { ticket = i; break; }
+    if (ticket < 0) return sfail(EGS_E_CAPACITY, "no free game slot");
+    const bool large = n >= s->large_n;
+    // cheap refusal before any packing: not even the smallest layout of this game fits now
+    const size_t least = large ? footprint_large(n, m, egs::pack_shift(n, 0) > 0) : footprint(n, m, 0);
+    if (!s->alloc.fits(least)) return sfail(EGS_E_CAPACITY, "arena full (%zu bytes wanted)", least);
+    SHIP(hipSetDevice(s->device));
+    if (large) return submit_large(s, ticket, n, n0, m, mg, ptr, succ, w);
+    // one pass over the input: checks, max |w|, 32-bit packing (int2 when w does not fit)
+    const size_t ptr_b = 4 * ((size_t)n + 1);
+    int sb = 0;
+    int rc = staging_reserve(s, al256(ptr_b) + 8 * (size_t)m, &sb);
+    if (rc) return rc;
+    char *stg = (char *)s->staging[sb];
+    const int shift_n = egs::pack_shift(n, 0);
     int32_t wm = 0, dm = 0;
     int64_t bad = -1;
-    switch (egs_pk_check(n, m, ptr, succ, w, &wm, &dm, &bad)) {
+    uint32_t *pe = (uint32_t *)(stg + al256(ptr_b));
+    const int chk = shift_n ? egs_pk_check_pack32(n, m, ptr, succ, w, shift_n, pe, &wm, &dm, &bad)
+                            : egs_pk_check(n, m, ptr, succ, w, &wm, &dm, &bad);
+    switch (chk) {
     case EGS_PK_BAD_OFFSETS: return sfail(EGS_E_ARG, "egs_server_submit: CSR offsets must run 0..m");
     case EGS_PK_NO_EDGE: return sfail(EGS_E_ARG, "egs_server_submit: vertex %lld has no out-edge", (long long)bad);
     case EGS_PK_BAD_SUCC:
         return sfail(EGS_E_ARG, "egs_server_submit: successor out of range at edge %lld", (long long)bad);
     default: break;
     }
-    int ticket = -1;
-    for (int i = 0; i < s->max_games; i++)
-        if (!s->slots[i].used) { ticket = i; break; }
-    if (ticket < 0) return sfail(EGS_E_CAPACITY, "no free game slot");
-    const bool large = n >= s->large_n;
-    const size_t bytes = large ? footprint_large(n, m) : footprint(n, m, wm);
+    const int shift = egs::pack_shift(n, wm);
+    if (!shift) egs_pk_pack64(m, succ, w, (int32_t *)pe);
+    egs_pk_copy(stg, ptr, ptr_b);
+    const size_t edge_b = (shift ? 4 : 8) * (size_t)m;
+    const size_t bytes = footprint(n, m, wm);
     size_t at = 0;
     if (!s->alloc.alloc(bytes, at)) return sfail(EGS_E_CAPACITY, "arena full (%zu bytes wanted)", bytes);
-    SHIP(hipSetDevice(s->device));
-    if (large) return submit_large(s, ticket, at, bytes, n, n0, m, mg, ptr, succ, w);
-    const int shift = egs::pack_shift(n, wm);
-    const size_t ptr_b = 4 * ((size_t)n + 1), edge_b = (shift ? 4 : 8) * (size_t)m;
-    int sb = 0;
-    int rc = staging_reserve(s, ptr_b + edge_b, &sb);
-    if (rc) {
-        s->alloc.release(at, bytes);
-        return rc;
-    }
-    char *stg = (char *)s->staging[sb];
-    egs_pk_copy(stg, ptr, ptr_b);
-    if (shift)
-        egs_pk_pack32(m, succ, w, shift, (uint32_t *)(stg + ptr_b));
-    else
-        egs_pk_pack64(m, succ, w, (int32_t *)(stg + ptr_b));
     char *base = (char *)s->arena + at;
     egs::GameDesc d{};
     d.ptr = (const int *)base;
@@ -488,7 +546,7 @@ int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t m
     d.shift = shift;
     d.group = 1;
     SHIP(hipMemcpyAsync((void *)d.ptr, stg, ptr_b, hipMemcpyHostToDevice, s->copy));
-    SHIP(hipMemcpyAsync((void *)d.edges, stg + ptr_b, edge_b, hipMemcpyHostToDevice, s->copy));
+    SHIP(hipMemcpyAsync((void *)d.edges, pe, edge_b, hipMemcpyHostToDevice, s->copy));
     SHIP(hipEventRecord(s->staging_ev[sb], s->copy));
     SHIP(hipMemsetAsync(s->d_states + ticket, 0, sizeof(egs::GameState), s->copy));
     SHIP(hipEventRecord(s->ev_upload, s->copy));

@@ -232,6 +232,21 @@ int grid_for(int64_t work, int threads, int cap) {
     return (int)b;
 }
 
+// 32-bit packed edges ((w << shift) | succ, the batched and serving layout) to int2.
+__global__ void k_unpack_edges(const uint32_t *__restrict__ packed, int shift, int2 *__restrict__ edges, int64_t m) {
+    const uint32_t mask = (1u << shift) - 1u;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = packed[e];
+        edges[e] = make_int2((int)(p & mask), ((int)p) >> shift);
+    }
+}
+
+hipError_t launch_unpack(const uint32_t *packed, int shift, int2 *edges, int64_t m, hipStream_t st) {
+    if (m > 0)
+        hipLaunchKernelGGL(k_unpack_edges, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st, packed, shift, edges, m);
+    return hipGetLastError();
+}
+
 hipError_t launch_pack(const int *succ, const int *w, int2 *edges, int64_t m, hipStream_t st) {
     if (m > 0)
         hipLaunchKernelGGL(k_pack_edges, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st, succ, w, edges, m);

@@ -1,4 +1,7 @@
 // egs_pack.cpp -- OpenMP-parallel host checks and packing of a game's CSR (egs_pack.h).
+//
+// The fast paths are single vectorisable passes with OR/MAX reductions; the index of an
+// offending vertex or edge is searched for only after a pass has seen one.
 #include "../egs_pack.h"
 
 #include <algorithm>
@@ -6,37 +9,123 @@
 #include <cstring>
 
 namespace {
-constexpr int64_t kParallelMin = 1 << 15;   // below this many edges one thread is faster
+constexpr int64_t kParallelMin = 1 << 15;   // below this many items one thread is faster
+
+// Degree check over the offsets: max out-degree, or the first vertex without an edge.
+int check_offsets(int32_t n, int64_t m, const int32_t *ptr, int32_t *maxdeg, int64_t *bad) {
+    if (ptr[0] != 0 || ptr[n] != m) return EGS_PK_BAD_OFFSETS;
+    int32_t dm = 0, dmin = INT32_MAX;
+#pragma omp parallel for if (n > kParallelMin) schedule(static) reduction(max : dm) reduction(min : dmin)
+    for (int32_t v = 0; v < n; v++) {
+        const int32_t d = ptr[v + 1] - ptr[v];
+        dm = std::max(dm, d);
+        dmin = std::min(dmin, d);
+    }
+    if (dmin < 1) {
+        for (int32_t v = 0; v < n; v++)
+            if (ptr[v + 1] - ptr[v] < 1) {
+                *bad = v;
+                break;
+            }
+        return EGS_PK_NO_EDGE;
+    }
+    *maxdeg = dm;
+    return EGS_PK_OK;
 }
+
+int64_t first_bad_succ(int32_t n, int64_t m, const int32_t *succ) {
+    for (int64_t e = 0; e < m; e++)
+        if ((uint32_t)succ[e] >= (uint32_t)n) return e;
+    return -1;
+}
+
+inline uint32_t absw(int32_t w) { return w < 0 ? 0u - (uint32_t)w : (uint32_t)w; }
+}  // namespace
 
 extern "C" int egs_pk_check(int32_t n, int64_t m, const int32_t *ptr, const int32_t *succ, const int32_t *w,
                             int32_t *wmax, int32_t *maxdeg, int64_t *bad) {
     *bad = -1;
-    if (ptr[0] != 0 || ptr[n] != m) return EGS_PK_BAD_OFFSETS;
-    int64_t first_v = INT64_MAX, first_e = INT64_MAX;
-    int32_t wm = 0, dm = 0;
-#pragma omp parallel for if (m > kParallelMin) schedule(static) reduction(min : first_v) reduction(max : dm)
-    for (int32_t v = 0; v < n; v++) {
-        const int32_t d = ptr[v + 1] - ptr[v];
-        if (d < 1 && v < first_v) first_v = v;
-        dm = std::max(dm, d);
-    }
-    if (first_v != INT64_MAX) {
-        *bad = first_v;
-        return EGS_PK_NO_EDGE;
-    }
-#pragma omp parallel for if (m > kParallelMin) schedule(static) reduction(min : first_e) reduction(max : wm)
+    int rc = check_offsets(n, m, ptr, maxdeg, bad);
+    if (rc) return rc;
+    uint32_t wm = 0, sbad = 0;
+#pragma omp parallel for if (m > kParallelMin) schedule(static) reduction(max : wm) reduction(| : sbad)
     for (int64_t e = 0; e < m; e++) {
-        if ((succ[e] < 0 || succ[e] >= n) && e < first_e) first_e = e;
-        const int64_t a = std::llabs((long long)w[e]);
-        wm = std::max(wm, (int32_t)std::min<int64_t>(a, INT32_MAX));
+        sbad |= (uint32_t)((uint32_t)succ[e] >= (uint32_t)n);
+        wm = std::max(wm, absw(w[e]));
     }
-    if (first_e != INT64_MAX) {
-        *bad = first_e;
+    if (sbad) {
+        *bad = first_bad_succ(n, m, succ);
         return EGS_PK_BAD_SUCC;
     }
-    *wmax = wm;
-    *maxdeg = dm;
+    *wmax = (int32_t)std::min<uint32_t>(wm, INT32_MAX);
+    return EGS_PK_OK;
+}
+
+extern "C" int egs_pk_check_pack32(int32_t n, int64_t m, const int32_t *ptr, const int32_t *succ,
+                                   const int32_t *w, int shift, uint32_t *out, int32_t *wmax, int32_t *maxdeg,
+                                   int64_t *bad) {
+    *bad = -1;
+    int rc = check_offsets(n, m, ptr, maxdeg, bad);
+    if (rc) return rc;
+    uint32_t wm = 0, sbad = 0;
+#pragma omp parallel for if (m > kParallelMin) schedule(static) reduction(max : wm) reduction(| : sbad)
+    for (int64_t e = 0; e < m; e++) {
+        const int32_t s = succ[e], x = w[e];
+        sbad |= (uint32_t)((uint32_t)s >= (uint32_t)n);
+        wm = std::max(wm, absw(x));
+        out[e] = ((uint32_t)x << shift) | (uint32_t)s;
+    }
+    if (sbad) {
+        *bad = first_bad_succ(n, m, succ);
+        return EGS_PK_BAD_SUCC;
+    }
+    *wmax = (int32_t)std::min<uint32_t>(wm, INT32_MAX);
+    return EGS_PK_OK;
+}
+
+extern "C" int egs_pk_check_pack64(int32_t n, int64_t m, const int32_t *ptr, const int32_t *succ,
+                                   const int32_t *w, int32_t *out, int32_t *wmax, int32_t *maxdeg, int64_t *bad) {
+    *bad = -1;
+    int rc = check_offsets(n, m, ptr, maxdeg, bad);
+    if (rc) return rc;
+    uint32_t wm = 0, sbad = 0;
+#pragma omp parallel for if (m > kParallelMin) schedule(static) reduction(max : wm) reduction(| : sbad)
+    for (int64_t e = 0; e < m; e++) {
+        const int32_t s = succ[e], x = w[e];
+        sbad |= (uint32_t)((uint32_t)s >= (uint32_t)n);
+        wm = std::max(wm, absw(x));
+        out[2 * e] = s;
+        out[2 * e + 1] = x;
+    }
+    if (sbad) {
+        *bad = first_bad_succ(n, m, succ);
+        return EGS_PK_BAD_SUCC;
+    }
+    *wmax = (int32_t)std::min<uint32_t>(wm, INT32_MAX);
+    return EGS_PK_OK;
+}
+
+extern "C" int egs_pk_check_copy(int32_t n, int64_t m, const int32_t *ptr, const int32_t *succ, const int32_t *w,
+                                 int32_t *out_ptr, int32_t *out_succ, int32_t *out_w, int32_t *wmax,
+                                 int32_t *maxdeg, int64_t *bad) {
+    *bad = -1;
+    int rc = check_offsets(n, m, ptr, maxdeg, bad);
+    if (rc) return rc;
+    egs_pk_copy(out_ptr, ptr, 4 * ((size_t)n + 1));
+    uint32_t wm = 0, sbad = 0;
+#pragma omp parallel for if (m > kParallelMin) schedule(static) reduction(max : wm) reduction(| : sbad)
+    for (int64_t e = 0; e < m; e++) {
+        const int32_t s = succ[e], x = w[e];
+        sbad |= (uint32_t)((uint32_t)s >= (uint32_t)n);
+        wm = std::max(wm, absw(x));
+        out_succ[e] = s;
+        out_w[e] = x;
+    }
+    if (sbad) {
+        *bad = first_bad_succ(n, m, succ);
+        return EGS_PK_BAD_SUCC;
+    }
+    *wmax = (int32_t)std::min<uint32_t>(wm, INT32_MAX);
     return EGS_PK_OK;
 }
 
@@ -54,7 +143,7 @@ extern "C" void egs_pk_pack64(int64_t m, const int32_t *succ, const int32_t *w, 
 }
 
 extern "C" void egs_pk_copy(void *dst, const void *src, size_t bytes) {
-    constexpr size_t kChunk = 1 << 20;
+    constexpr size_t kChunk = 64 << 10;
     const int64_t chunks = (int64_t)((bytes + kChunk - 1) / kChunk);
 #pragma omp parallel for if (bytes > 4 * kChunk) schedule(static)
     for (int64_t c = 0; c < chunks; c++) {

@@ -133,6 +133,12 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt);
 int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st);
 
 int egs_solver_stats(egs_solver *s, egs_stats *st);
+/* No reference counterpart.  Load a game already resident on the device in the 32-bit
+ * packed edge layout of the batched and serving paths (edge e = (w << shift) | succ,
+ * 1 <= shift <= 30), with its max |w| and max out-degree known; the CSR is trusted to
+ * have been checked (every vertex has an edge, successors in range). */
+int egs_solver_load_packed(egs_solver *s, int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *d_ptr,
+                           const uint32_t *d_edges, int32_t shift, int32_t wmax, int32_t max_outdeg);
 int egs_solver_read_values(egs_solver *s, int32_t *host_values);
 const int32_t *egs_solver_device_values(egs_solver *s);
 /* Device arrays of the transposed graph (for inspection): cptr[n+1], cpred[m]. */

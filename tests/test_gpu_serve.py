@@ -105,3 +105,24 @@ def test_served_large_route_exact(oracle):
         vals, st = srv.fetch(t)
         assert np.array_equal(vals, lfp(oracle, to_oracle_game(reqs[0].game)))
         assert st["sweeps"] >= 1
+
+
+def test_served_large_route_wide_weights(oracle):
+    """Large-route games whose weights do not fit beside a packed successor id travel as
+    separate successor and weight arrays; small-weight ones as packed 32-bit edges."""
+    import egsolver_amd as E
+    games = [E.generate(E.GenParams(n=3000, seed=31, deg_kind=1, deg_lo=2, deg_hi=6, wmax=600000)),
+             E.generate(E.GenParams(n=2500, seed=32, deg_kind=1, deg_lo=2, deg_hi=6, wmax=50))]
+    _say(f"large route wide/narrow weights, mg {[g.mg for g in games]}")
+    arena = sum(E.Server.footprint(g) for g in games) + (1 << 20)
+    with E.Server(arena, max_games=4) as srv:
+        srv.set_route(1, 2)
+        tickets = [srv.submit(g) for g in games]
+        assert None not in tickets
+        got = {}
+        while len(got) < len(tickets):
+            srv.launch()
+            for t in srv.wait():
+                got[t], _ = srv.fetch(t)
+        for t, g in zip(tickets, games):
+            assert np.array_equal(got[t], lfp(oracle, to_oracle_game(g)))
