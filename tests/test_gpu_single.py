@@ -198,3 +198,20 @@ def test_phased_unpacked_edges_equal_single_sweep():
             assert conv and s.violations() == 0
             out.append(s.values())
         assert np.array_equal(out[0], out[1])
+
+
+def test_config3_full_size_fixpoint():
+    """The skewed-degree game at full size (n = 2^20, power-law out-degrees up to 10^5):
+    fixpoint certificate, and phased vs single-sweep dense rounds agree."""
+    import egsolver_amd as E
+    g = E.config_game(3)
+    _say(f"config 3 full size n {g.n} m {g.m}")
+    with E.Solver(g.n, g.m) as s:
+        s.load(g)
+        out = []
+        for sl in (0, 1):
+            s.prepare(E.options(slices=sl))
+            conv, st = s.iterate()
+            assert conv and s.violations() == 0
+            out.append(s.values())
+        assert np.array_equal(out[0], out[1])
