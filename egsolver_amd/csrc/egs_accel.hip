@@ -57,9 +57,9 @@ __global__ __launch_bounds__(256) void k_gap_hist(DevGame g, int wmax, int *__re
     }
 }
 
-// D membership: a gap threshold (B >= 0) or the vertices that rose in round `last` (B < 0).
-__device__ __forceinline__ bool in_d(const DevGame &g, int v, int B, int last) {
-    const int x = g.val[v];
+// D membership of a vertex whose value x is already loaded: a gap threshold (B >= 0) or
+// the vertices that rose in round `last` (B < 0).
+__device__ __forceinline__ bool in_d_x(const DevGame &g, int v, int x, int B, int last) {
     return x != EGS_TOP && (B >= 0 ? x > B : g.rise[v] == last);
 }
 
@@ -73,39 +73,98 @@ __device__ __forceinline__ int wave_sum(int x) {
 constexpr long long kHOut = LLONG_MIN;        // not in D, not TOP
 constexpr long long kHTop = LLONG_MIN + 1;    // TOP: a successor that ends the path
 
-// Closure check (owner-0 members: every successor in D or TOP; owner-1: at least one),
-// member count, and h for every vertex: the warm start (n+1) (MG - f) on D, else a sentinel.
-__global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, long long *__restrict__ h,
+// Pass counters live in a ring of kCntSlots ints: pass p adds its raises to slot p % kCntSlots
+// and zeroes slot (p + 1) % kCntSlots for the pass after it.
+constexpr int kCntSlots = 64;
+
+// Closure check, member count and the first pass of the longest-path game, in one sweep
+// over the values.  The first pass needs no potentials: the warm start (n+1) (MG - f) of a
+// successor follows from its value, so it is computed here from f (a Jacobi pass from the
+// warm start).  Writes h for every vertex (the first pass's result on D, else a sentinel),
+// the raised vertices' bits into `bits` (every word of the vertex range is stored), zeroes
+// `clear` (the bitmap of the next pass), and counts raises into cnt[0].
+__global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, long long bound,
+                                                 long long *__restrict__ h, unsigned *__restrict__ bits,
+                                                 unsigned *__restrict__ clear, int *__restrict__ cnt,
                                                  int *__restrict__ flags) {
     const long long scale = (long long)g.n + 1;
-    int members = 0;
+    const int lane = threadIdx.x & 63;
+    int members = 0, raised = 0;
+    // the stride is a multiple of 64, so a wave always covers 64 aligned vertices (two words)
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
         const int x = g.val[v];
-        if (x == EGS_TOP) { h[v] = kHTop; continue; }
-        if (!in_d(g, (int)v, B, last)) { h[v] = kHOut; continue; }
-        members++;
-        h[v] = scale * ((long long)g.mg - x);
-        bool any = false, all = true;
-        for (int k = g.ptr[v]; k < g.ptr[v + 1]; k++) {
-            const int u = g.edges[k].x;
-            const bool ok = g.val[u] == EGS_TOP || in_d(g, u, B, last);
-            any |= ok;
-            all &= ok;
+        long long hv;
+        bool up = false;
+        if (x == EGS_TOP) {
+            hv = kHTop;
+        } else if (!in_d_x(g, (int)v, x, B, last)) {
+            hv = kHOut;
+        } else {
+            members++;
+            const long long h0 = scale * ((long long)g.mg - x);
+            const bool owner1 = v >= g.n0;
+            long long best = owner1 ? LLONG_MAX : LLONG_MIN;
+            bool any = false, all = true;
+            for (int k = g.ptr[v]; k < g.ptr[v + 1]; k++) {
+                const int2 ew = g.edges[k];
+                const int xu = g.val[ew.x];
+                if (xu == EGS_TOP) {                 // ends the path: -infinity for the game
+                    any = true;
+                    if (owner1) best = LLONG_MIN;
+                } else if (in_d_x(g, ew.x, xu, B, last)) {
+                    any = true;
+                    const long long c = scale * ew.y + 1 + scale * ((long long)g.mg - xu);
+                    best = owner1 ? min(best, c) : max(best, c);
+                } else {
+                    all = false;
+                }
+            }
+            if (v >= g.n0 ? !any : !all) atomicOr(&flags[0], 1);
+            hv = h0;
+            if (best != LLONG_MAX && best > h0) {
+                hv = best;
+                up = true;
+                raised++;
+                if (best > bound) atomicOr(&flags[1], 1);
+            }
         }
-        if (v >= g.n0 ? !any : !all) atomicOr(&flags[0], 1);
+        h[v] = hv;
+        const unsigned long long bal = __ballot(up);
+        if (lane == 0 || lane == 32) {
+            const int64_t word = v >> 5;
+            bits[word] = (unsigned)(bal >> lane);
+            clear[word] = 0u;
+        }
     }
     members = wave_sum(members);
-    if ((threadIdx.x & 63) == 0 && members) atomicAdd(&flags[3], members);
+    raised = wave_sum(raised);
+    if (lane == 0 && members) atomicAdd(&flags[3], members);
+    if (lane == 0 && raised) atomicAdd(&cnt[0], raised);
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[1] = 0;
 }
 
-// One in-place pass of the longest-path game on D, G lanes per vertex: owner 0 takes the
+// Pass p >= 1 of the longest-path game on D, in place, G lanes per vertex: owner 0 takes the
 // max over its edges into D, owner 1 the min over its edges into D or TOP (a TOP successor
-// ends the path); counts raised entries.
+// ends the path).  Only vertices with a successor raised in pass p-1 (bit set in `prev`) are
+// evaluated: a vertex none of whose successors rose since its last evaluation keeps its
+// potential, since every raise in pass q is seen by the evaluations of pass q+1 (a kernel
+// boundary lies between).  So a pass that raises nothing leaves a fixpoint.  Raised vertices
+// set their bit in `cur`; `clear` (the bitmap of pass p+1, last read in pass p-1) is zeroed.
+// The pass returns at once when pass p-1 raised nothing or the attempt has already failed.
 template <int G>
 __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restrict__ h, long long bound,
-                                                 int *__restrict__ changed, int *__restrict__ flags) {
+                                                 const unsigned *__restrict__ prev, unsigned *__restrict__ cur,
+                                                 unsigned *__restrict__ clear, int p, int *__restrict__ cnt,
+                                                 int *__restrict__ flags) {
+    if (cnt[(p - 1) % kCntSlots] == 0 || (flags[0] | flags[1])) return;
+    const int64_t words = ((int64_t)g.n + 31) >> 5;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+        clear[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[(p + 1) % kCntSlots] = 0;
     constexpr int kPer = 256 / G;
+    constexpr unsigned long long kGroupMask = (G == 64) ? ~0ull : ((1ull << G) - 1);
     const int gl = threadIdx.x % G;
+    const int gbase = (threadIdx.x & 63) & ~(G - 1);
     const long long scale = (long long)g.n + 1;
     int local = 0;
     for (int64_t base = (int64_t)blockIdx.x * kPer; base < g.n; base += (int64_t)gridDim.x * kPer) {
@@ -113,10 +172,20 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
         const long long hv = v < g.n ? h[v] : kHOut;
         const bool member = hv >= 0;
         const bool owner1 = v >= g.n0;
-        long long best = owner1 ? LLONG_MAX : LLONG_MIN;
+        int kb = 0, ke = 0;
+        bool need = false;
         if (member) {
-            const int e = g.ptr[v + 1];
-            for (int k = g.ptr[v] + gl; k < e; k += G) {
+            kb = g.ptr[v];
+            ke = g.ptr[v + 1];
+            for (int k = kb + gl; k < ke && !need; k += G) {
+                const int u = g.edges[k].x;
+                need = (prev[u >> 5] >> (u & 31)) & 1u;
+            }
+        }
+        need = ((__ballot(need) >> gbase) & kGroupMask) != 0;
+        long long best = owner1 ? LLONG_MAX : LLONG_MIN;
+        if (member && need) {
+            for (int k = kb + gl; k < ke; k += G) {
                 const int2 ew = g.edges[k];
                 const long long hu = h[ew.x];
                 if (hu == kHOut) continue;
@@ -129,14 +198,15 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
             const long long o = __shfl_xor(best, d, 64);
             best = owner1 ? min(best, o) : max(best, o);
         }
-        if (member && gl == 0 && best != LLONG_MAX && best > hv) {
+        if (member && need && gl == 0 && best != LLONG_MAX && best > hv) {
             h[v] = best;
+            atomicOr(&cur[v >> 5], 1u << (v & 31));
             local++;
             if (best > bound) atomicOr(&flags[1], 1);
         }
     }
     local = wave_sum(local);
-    if ((threadIdx.x & 63) == 0 && local) atomicAdd(changed, local);
+    if ((threadIdx.x & 63) == 0 && local) atomicAdd(&cnt[p % kCntSlots], local);
 }
 
 // D is where h >= 0 (set by k_closure for this candidate).
@@ -150,13 +220,13 @@ __global__ __launch_bounds__(256) void k_mark_top(DevGame g, const long long *__
 }
 
 static void launch_bf_pass(int G, int grid, hipStream_t st, const DevGame &g, long long *h, long long bound,
-                           int *changed, int *flags) {
+                           const unsigned *prev, unsigned *cur, unsigned *clear, int p, int *cnt, int *flags) {
     switch (G) {
-    case 1: hipLaunchKernelGGL(k_bf_pass<1>, dim3(grid), dim3(256), 0, st, g, h, bound, changed, flags); break;
-    case 2: hipLaunchKernelGGL(k_bf_pass<2>, dim3(grid), dim3(256), 0, st, g, h, bound, changed, flags); break;
-    case 4: hipLaunchKernelGGL(k_bf_pass<4>, dim3(grid), dim3(256), 0, st, g, h, bound, changed, flags); break;
-    case 8: hipLaunchKernelGGL(k_bf_pass<8>, dim3(grid), dim3(256), 0, st, g, h, bound, changed, flags); break;
-    default: hipLaunchKernelGGL(k_bf_pass<16>, dim3(grid), dim3(256), 0, st, g, h, bound, changed, flags); break;
+    case 1: hipLaunchKernelGGL(k_bf_pass<1>, dim3(grid), dim3(256), 0, st, g, h, bound, prev, cur, clear, p, cnt, flags); break;
+    case 2: hipLaunchKernelGGL(k_bf_pass<2>, dim3(grid), dim3(256), 0, st, g, h, bound, prev, cur, clear, p, cnt, flags); break;
+    case 4: hipLaunchKernelGGL(k_bf_pass<4>, dim3(grid), dim3(256), 0, st, g, h, bound, prev, cur, clear, p, cnt, flags); break;
+    case 8: hipLaunchKernelGGL(k_bf_pass<8>, dim3(grid), dim3(256), 0, st, g, h, bound, prev, cur, clear, p, cnt, flags); break;
+    default: hipLaunchKernelGGL(k_bf_pass<16>, dim3(grid), dim3(256), 0, st, g, h, bound, prev, cur, clear, p, cnt, flags); break;
     }
 }
 
@@ -218,11 +288,15 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
     int ncand = gap_candidates(cb.pinned, cb.pinned + kBins, wmax, g.mg, cands, std::min(max_cands, 40));
     cands[ncand++] = -1;                    // the vertices that rose in round `last`
     int *flags = cb.hist + 2 * kBins;       // [0] closure failed, [1] bound exceeded, [2] marked, [3] members
-    int *cnt = cb.pinned + 2 * kBins;       // up to 8 pass counters
-    int *fl = cnt + 8;                      // flags readback
-    // convergence is read back every `chunk` passes: often on large games, whose passes
-    // cost far more than the read-back
-    const int chunk = g.n >= (1 << 20) ? 2 : 8;
+    int *fl = cb.pinned + 2 * kBins;        // flags readback (4 ints), then the counter ring
+    int *cnt = fl + 4;
+    const int64_t words = ((int64_t)g.n + 31) >> 5;
+    unsigned *bm[3] = {cb.bits, cb.bits + words, cb.bits + 2 * words};
+    // Passes are queued in chunks and read back once per chunk; a pass after the converged
+    // one returns at once, so a chunk costs little beyond its real passes.  The first
+    // chunk (closure and first pass, then more) is sized for the common case of a few
+    // passes; later chunks are longer.
+    const int first_chunk = 6, chunk = 8;
     int G = 1;
     while (G < 16 && G < (double)g.m / g.n) G <<= 1;
     const int bf_grid = grid_for((int64_t)g.n * G, 256, 4096);
@@ -231,25 +305,31 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
         res.threshold = B;
         res.tried++;
         if (hipMemsetAsync(flags, 0, sizeof(int) * 4, st) != hipSuccess) return -1;
-        hipLaunchKernelGGL(k_closure, dim3(grid), dim3(256), 0, st, g, B, last, cb.h, flags);
-        if (hipMemcpyAsync(fl, flags, sizeof(int) * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
-        if (hipStreamSynchronize(st) != hipSuccess) return -1;
-        if (fl[3] == 0) continue;
-        if (fl[0]) { res.closure_failed++; continue; }
+        if (hipMemsetAsync(cb.bfcnt, 0, sizeof(int) * kCntSlots, st) != hipSuccess) return -1;
+        hipLaunchKernelGGL(k_closure, dim3(grid), dim3(256), 0, st, g, B, last, bound, cb.h, bm[0], bm[1], cb.bfcnt,
+                           flags);
+        int queued = 1;                     // passes queued so far: pass 0 is the closure's
         int passes = 0;
-        bool converged = false, diverged = false;
-        while (!converged && !diverged && passes < cap) {
-            if (hipMemsetAsync(cb.bfcnt, 0, sizeof(int) * chunk, st) != hipSuccess) return -1;
-            for (int k = 0; k < chunk; k++) launch_bf_pass(G, bf_grid, st, g, cb.h, bound, cb.bfcnt + k, flags);
-            if (hipMemcpyAsync(cnt, cb.bfcnt, sizeof(int) * chunk, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
-            if (hipMemcpyAsync(fl, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+        bool converged = false, diverged = false, rejected = false;
+        while (!converged && !diverged && !rejected && passes < cap) {
+            const int upto = std::min(cap, queued == 1 ? first_chunk : queued + chunk);
+            for (int p = queued; p < upto; p++)
+                launch_bf_pass(G, bf_grid, st, g, cb.h, bound, bm[(p - 1) % 3], bm[p % 3], bm[(p + 1) % 3], p, cb.bfcnt,
+                               flags);
+            if (hipMemcpyAsync(fl, flags, sizeof(int) * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+            if (hipMemcpyAsync(cnt, cb.bfcnt, sizeof(int) * kCntSlots, hipMemcpyDeviceToHost, st) != hipSuccess)
+                return -1;
             if (hipStreamSynchronize(st) != hipSuccess) return -1;
+            if (fl[3] == 0) { rejected = true; break; }
+            if (fl[0]) { res.closure_failed++; rejected = true; break; }
             if (fl[1]) diverged = true;
-            for (int k = 0; k < chunk && !diverged; k++) {
-                passes++;
-                if (cnt[k] == 0) { converged = true; break; }
+            for (int p = queued - (queued > 1 ? 0 : 1); p < upto && !diverged; p++) {
+                passes = p + 1;
+                if (cnt[p % kCntSlots] == 0) { converged = true; break; }
             }
+            queued = upto;
         }
+        if (rejected) continue;
         res.passes += passes;
         if (!converged) {
             if (!diverged) res.cap_hit = 1;

@@ -63,7 +63,7 @@ constexpr int kPinConv = 448;     // kListCounters ints
 constexpr int kPinRounds = 576;
 constexpr int kPinStats = 640;    // 4 x u64
 constexpr int kPinViol = 704;     // 1 x u64
-constexpr int kPinCert = 1024;    // certification readbacks (2 x 64 + 16 ints)
+constexpr int kPinCert = 1024;    // certification readbacks (2 x 64 + 4 + 64 ints)
 constexpr int kPinForce = 1280;   // round number forced to a full sweep
 constexpr int kPinOne = 1281;     // the constant 1 (raises ring[12])
 constexpr int kProbeWant = 100;   // offset in a probe slot: ring[13] (a round wanted to go sparse)
@@ -93,6 +93,7 @@ struct egs_solver {
     int *d_ring = nullptr, *d_rounds = nullptr, *d_misc = nullptr, *d_tile = nullptr, *d_fill = nullptr;
     int sub_cap = 0;
     int *d_sigma = nullptr, *d_hist = nullptr, *d_bfcnt = nullptr;   // d_sigma: rise stamps
+    unsigned *d_bits = nullptr;     // three bitmaps of raised potentials (proof passes)
     int *d_sptr = nullptr, *d_acc = nullptr, *d_stile = nullptr;    // phased dense rounds (large n_cap)
     int2 *d_sedges = nullptr;
     unsigned long long *d_bigmask = nullptr;
@@ -224,7 +225,8 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
            o_ring = take(4 * 16), o_rounds = take(4 * 16), o_misc = take(8 * 16),
            o_tile = take(4 * (size_t)tiles), o_fill = take(4 * n1),
            o_stats = take(8 * 4 * (size_t)egs::kStatSlots), o_statsum = take(8 * 8),
-           o_sigma = take(4 * n1), o_h = take(8 * n1), o_hist = take(4 * 256), o_bfcnt = take(4 * 64);
+           o_sigma = take(4 * n1), o_h = take(8 * n1), o_hist = take(4 * 256), o_bfcnt = take(4 * 64),
+           o_bits = take(4 * 3 * (n1 / 32 + 1));
     // phased dense rounds: slice offsets, slice-major edges, partial keys, scan tiles
     const bool phased = n_cap >= egs::kPhaseMinN;
     const size_t pn1 = (size_t)egs::kPhases * n_cap + 1;
@@ -262,6 +264,7 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
     s->d_h = (long long *)(b + o_h);
     s->d_hist = (int *)(b + o_hist);
     s->d_bfcnt = (int *)(b + o_bfcnt);
+    s->d_bits = (unsigned *)(b + o_bits);
     if (phased) {
         s->d_sptr = (int *)(b + o_sptr);
         s->d_sedges = (int2 *)(b + o_sedges);
@@ -517,7 +520,7 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
                                    sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
             HIP_TRY(hipStreamSynchronize(s->stream));
             if (list_total(s, s->h_probe + kPinConv) == 0) break;
-            egs::CertBuffers cb{s->d_hist, s->d_h, s->d_bfcnt, s->h_probe + kPinCert};
+            egs::CertBuffers cb{s->d_hist, s->d_h, s->d_bfcnt, s->d_bits, s->h_probe + kPinCert};
             egs::CertResult cr;
             const int got = egs::certify_top(s->g, cb, s->wmax, (int)(s->next_round - 1), s->cert_cap,
                                              kCertCandidates, s->stream, cr);

@@ -88,8 +88,9 @@ int grid_for(int64_t work, int threads, int cap);
 struct CertBuffers {
     int *hist;        // device: 2 x 64 histogram bins + 8 flags
     long long *h;     // device: n longest-path potentials
-    int *bfcnt;       // device: 8 pass counters
-    int *pinned;      // host pinned: 2 x 64 + 16 ints
+    int *bfcnt;       // device: ring of 64 pass counters
+    unsigned *bits;   // device: 3 bitmaps of ceil(n / 32) words (vertices raised per pass)
+    int *pinned;      // host pinned: 2 x 64 + 4 + 64 ints
 };
 struct CertResult {
     int threshold;       // last threshold tried
