@@ -62,6 +62,7 @@ class EgsStats(ctypes.Structure):
         ("certificates", ctypes.c_int64),
         ("cert_passes", ctypes.c_int64),
         ("top_marked", ctypes.c_int64),
+        ("cert_pruned", ctypes.c_int64),
     ]
 
     def as_dict(self):

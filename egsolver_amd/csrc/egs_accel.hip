@@ -7,8 +7,9 @@
 //   1. Candidate D.  Either a gap threshold B (no vertex value in (B - W, B], W = max |w|,
 //      some non-TOP value above B, B <= MG - W) with D = {v : B < f(v) < TOP}, or the
 //      vertices that rose in the last round.  The candidate only decides what is tried.
-//   2. Closure, checked: every successor of an owner-0 vertex of D lies in D or is TOP;
-//      some successor of an owner-1 vertex of D does.
+//   2. Closure: every successor of an owner-0 vertex of D lies in D or is TOP; some
+//      successor of an owner-1 vertex of D does.  Vertices that break it are removed until
+//      it holds (the greatest closed subset of the candidate), during the passes of step 3.
 //   3. The longest-path game on D with w' = (n+1) w + 1:
 //        h(v) <- max(h(v), max_{u in D} w'(v,u) + h(u))                 (owner 0)
 //        h(v) <- max(h(v), min_{u in D or TOP} w'(v,u) + h(u))          (owner 1)
@@ -78,18 +79,22 @@ constexpr long long kHTop = LLONG_MIN + 1;    // TOP: a successor that ends the 
 constexpr int kCntSlots = 64;
 
 // Closure check, member count and the first pass of the longest-path game, in one sweep
-// over the values.  The first pass needs no potentials: the warm start (n+1) (MG - f) of a
-// successor follows from its value, so it is computed here from f (a Jacobi pass from the
-// warm start).  Writes h for every vertex (the first pass's result on D, else a sentinel),
-// the raised vertices' bits into `bits` (every word of the vertex range is stored), zeroes
-// `clear` (the bitmap of the next pass), and counts raises into cnt[0].
+// over the values.  A member that breaks closure (an owner-0 edge leaving D and not to TOP,
+// or an owner-1 vertex with no edge into D or TOP) is removed from D; the removal is a
+// change like a raise, so its predecessors are checked again in the next pass, and the
+// passes together compute the greatest closed subset of the candidate.  The first pass needs
+// no potentials: the warm start (n+1) (MG - f) of a successor follows from its value, so it
+// is computed here from f (a Jacobi pass from the warm start).  Writes h for every vertex
+// (the first pass's result on D, else a sentinel), the changed vertices' bits into `bits`
+// (every word of the vertex range is stored), zeroes `clear` (the bitmap of the next pass),
+// counts changes into cnt[0], members into flags[3] and removals into flags[0].
 __global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, long long bound,
                                                  long long *__restrict__ h, unsigned *__restrict__ bits,
                                                  unsigned *__restrict__ clear, int *__restrict__ cnt,
                                                  int *__restrict__ flags) {
     const long long scale = (long long)g.n + 1;
     const int lane = threadIdx.x & 63;
-    int members = 0, raised = 0;
+    int members = 0, changed = 0, removed = 0;
     // the stride is a multiple of 64, so a wave always covers 64 aligned vertices (two words)
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
         const int x = g.val[v];
@@ -99,6 +104,9 @@ __global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, lon
             hv = kHTop;
         } else if (!in_d_x(g, (int)v, x, B, last)) {
             hv = kHOut;
+        } else if (g.ptr[v + 1] - g.ptr[v] > g.big_deg) {
+            members++;                               // decided by k_closure_big
+            hv = scale * ((long long)g.mg - x);
         } else {
             members++;
             const long long h0 = scale * ((long long)g.mg - x);
@@ -119,16 +127,20 @@ __global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, lon
                     all = false;
                 }
             }
-            if (v >= g.n0 ? !any : !all) atomicOr(&flags[0], 1);
-            hv = h0;
-            if (best != LLONG_MAX && best > h0) {
+            if (owner1 ? !any : !all) {
+                hv = kHOut;
+                up = true;
+                removed++;
+            } else if (best > h0) {
                 hv = best;
                 up = true;
-                raised++;
                 if (best > bound) atomicOr(&flags[1], 1);
+            } else {
+                hv = h0;
             }
         }
         h[v] = hv;
+        changed += up;
         const unsigned long long bal = __ballot(up);
         if (lane == 0 || lane == 32) {
             const int64_t word = v >> 5;
@@ -137,26 +149,91 @@ __global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, lon
         }
     }
     members = wave_sum(members);
-    raised = wave_sum(raised);
+    changed = wave_sum(changed);
+    removed = wave_sum(removed);
     if (lane == 0 && members) atomicAdd(&flags[3], members);
-    if (lane == 0 && raised) atomicAdd(&cnt[0], raised);
+    if (lane == 0 && removed) atomicAdd(&flags[0], removed);
+    if (lane == 0 && changed) atomicAdd(&cnt[0], changed);
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[1] = 0;
+}
+
+constexpr int kBigU = 4;   // edges in flight per lane in the workgroup-per-vertex kernels
+
+// k_closure for the vertices above big_deg (launched after it), one workgroup each: the same
+// closure test and first pass; changes set their bit with an atomic OR, since k_closure has
+// stored every word already.
+__global__ __launch_bounds__(256) void k_closure_big(DevGame g, int B, int last, long long bound,
+                                                     long long *__restrict__ h, unsigned *__restrict__ bits,
+                                                     int *__restrict__ cnt, int *__restrict__ flags) {
+    __shared__ long long sm[4];
+    const long long scale = (long long)g.n + 1;
+    const int nb = g.ring[9], items = nb + g.ring[10];
+    for (int64_t i = blockIdx.x; i < items; i += gridDim.x) {
+        const int v = egs_big_item(g.sbig, g.shuge, nb, items - 1 - i);   // huge vertices first
+        const int x = g.val[v];
+        if (!in_d_x(g, v, x, B, last)) continue;        // the whole workgroup: v is shared
+        const bool owner1 = v >= g.n0;
+        long long best = owner1 ? LLONG_MAX : LLONG_MIN;
+        bool any = false, all = true;
+        const int e = g.ptr[v + 1];
+        for (int k0 = g.ptr[v] + (int)threadIdx.x; k0 < e; k0 += kBigU * 256) {
+            int2 ew[kBigU];
+            int xu[kBigU];
+#pragma unroll
+            for (int j = 0; j < kBigU; j++) ew[j] = k0 + 256 * j < e ? g.edges[k0 + 256 * j] : make_int2(0, 0);
+#pragma unroll
+            for (int j = 0; j < kBigU; j++) xu[j] = g.val[ew[j].x];
+#pragma unroll
+            for (int j = 0; j < kBigU; j++) {
+                if (k0 + 256 * j >= e) break;
+                if (xu[j] == EGS_TOP) {
+                    any = true;
+                    if (owner1) best = LLONG_MIN;
+                } else if (in_d_x(g, ew[j].x, xu[j], B, last)) {
+                    any = true;
+                    const long long c = scale * ew[j].y + 1 + scale * ((long long)g.mg - xu[j]);
+                    best = owner1 ? min(best, c) : max(best, c);
+                } else {
+                    all = false;
+                }
+            }
+        }
+        any = __syncthreads_or(any);
+        all = __syncthreads_and(all);
+        best = egs_block_minmax_ll(best, owner1, sm);
+        if (threadIdx.x == 0) {
+            const long long h0 = scale * ((long long)g.mg - x);
+            if (owner1 ? !any : !all) {
+                h[v] = kHOut;
+                atomicOr(&bits[v >> 5], 1u << (v & 31));
+                atomicAdd(&cnt[0], 1);
+                atomicAdd(&flags[0], 1);
+            } else if (best > h0) {
+                h[v] = best;
+                atomicOr(&bits[v >> 5], 1u << (v & 31));
+                atomicAdd(&cnt[0], 1);
+                if (best > bound) atomicOr(&flags[1], 1);
+            }
+        }
+    }
 }
 
 // Pass p >= 1 of the longest-path game on D, in place, G lanes per vertex: owner 0 takes the
 // max over its edges into D, owner 1 the min over its edges into D or TOP (a TOP successor
-// ends the path).  Only vertices with a successor raised in pass p-1 (bit set in `prev`) are
-// evaluated: a vertex none of whose successors rose since its last evaluation keeps its
-// potential, since every raise in pass q is seen by the evaluations of pass q+1 (a kernel
-// boundary lies between).  So a pass that raises nothing leaves a fixpoint.  Raised vertices
-// set their bit in `cur`; `clear` (the bitmap of pass p+1, last read in pass p-1) is zeroed.
-// The pass returns at once when pass p-1 raised nothing or the attempt has already failed.
+// ends the path); a member that no longer meets closure (its successors left D) is removed
+// instead.  Only vertices with a successor changed (raised or removed) in pass p-1, bit set
+// in `prev`, are evaluated: a vertex none of whose successors changed since its last
+// evaluation keeps its potential and its closure, since every change in pass q is seen by
+// the evaluations of pass q+1 (a kernel boundary lies between).  So a pass that changes
+// nothing leaves a closed D and a fixpoint on it.  Changed vertices set their bit in `cur`;
+// `clear` (the bitmap of pass p+1, last read in pass p-1) is zeroed.  The pass returns at
+// once when pass p-1 changed nothing or the potentials have passed the divergence bound.
 template <int G>
 __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restrict__ h, long long bound,
                                                  const unsigned *__restrict__ prev, unsigned *__restrict__ cur,
                                                  unsigned *__restrict__ clear, int p, int *__restrict__ cnt,
                                                  int *__restrict__ flags) {
-    if (cnt[(p - 1) % kCntSlots] == 0 || (flags[0] | flags[1])) return;
+    if (cnt[(p - 1) % kCntSlots] == 0 || flags[1]) return;
     const int64_t words = ((int64_t)g.n + 31) >> 5;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
         clear[i] = 0u;
@@ -177,6 +254,7 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
         if (member) {
             kb = g.ptr[v];
             ke = g.ptr[v + 1];
+            if (ke - kb > g.big_deg) ke = kb;         // k_bf_big evaluates it: no need here
             for (int k = kb + gl; k < ke && !need; k += G) {
                 const int u = g.edges[k].x;
                 need = (prev[u >> 5] >> (u & 31)) & 1u;
@@ -184,29 +262,101 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
         }
         need = ((__ballot(need) >> gbase) & kGroupMask) != 0;
         long long best = owner1 ? LLONG_MAX : LLONG_MIN;
+        bool out = false;                    // an edge leaves D and does not reach TOP
         if (member && need) {
             for (int k = kb + gl; k < ke; k += G) {
                 const int2 ew = g.edges[k];
                 const long long hu = h[ew.x];
-                if (hu == kHOut) continue;
+                if (hu == kHOut) { out = true; continue; }
                 const long long c = hu == kHTop ? LLONG_MIN : scale * ew.y + 1 + hu;
                 best = owner1 ? min(best, c) : max(best, c);
             }
         }
+        out = ((__ballot(out) >> gbase) & kGroupMask) != 0;
 #pragma unroll
         for (int d = G / 2; d >= 1; d >>= 1) {
             const long long o = __shfl_xor(best, d, 64);
             best = owner1 ? min(best, o) : max(best, o);
         }
-        if (member && need && gl == 0 && best != LLONG_MAX && best > hv) {
-            h[v] = best;
-            atomicOr(&cur[v >> 5], 1u << (v & 31));
-            local++;
-            if (best > bound) atomicOr(&flags[1], 1);
+        if (member && need && gl == 0) {
+            // owner 1 breaks closure when no edge reaches D or TOP, owner 0 when any leaves
+            if (owner1 ? best == LLONG_MAX : out) {
+                h[v] = kHOut;
+                atomicOr(&cur[v >> 5], 1u << (v & 31));
+                local++;
+            } else if (best > hv) {
+                h[v] = best;
+                atomicOr(&cur[v >> 5], 1u << (v & 31));
+                local++;
+                if (best > bound) atomicOr(&flags[1], 1);
+            }
         }
     }
     local = wave_sum(local);
     if ((threadIdx.x & 63) == 0 && local) atomicAdd(&cnt[p % kCntSlots], local);
+}
+
+// Pass p for the vertices above big_deg (launched after k_bf_pass for the same p), one
+// workgroup each, with the same rule and updates.
+__global__ __launch_bounds__(256) void k_bf_big(DevGame g, long long *__restrict__ h, long long bound,
+                                                const unsigned *__restrict__ prev, unsigned *__restrict__ cur,
+                                                int p, int *__restrict__ cnt, int *__restrict__ flags) {
+    __shared__ long long sm[4];
+    if (cnt[(p - 1) % kCntSlots] == 0 || flags[1]) return;
+    const long long scale = (long long)g.n + 1;
+    const int nb = g.ring[9], items = nb + g.ring[10];
+    int local = 0;
+    for (int64_t i = blockIdx.x; i < items; i += gridDim.x) {
+        const int v = egs_big_item(g.sbig, g.shuge, nb, items - 1 - i);   // huge vertices first
+        const long long hv = h[v];
+        if (hv < 0) continue;                           // not in D: the whole workgroup
+        const int kb = g.ptr[v], ke = g.ptr[v + 1];
+        bool need = false;
+        for (int k0 = kb + (int)threadIdx.x; k0 < ke && !need; k0 += kBigU * 256) {
+            unsigned wd[kBigU];
+            int u[kBigU];
+#pragma unroll
+            for (int j = 0; j < kBigU; j++) u[j] = k0 + 256 * j < ke ? g.edges[k0 + 256 * j].x : 0;
+#pragma unroll
+            for (int j = 0; j < kBigU; j++) wd[j] = prev[u[j] >> 5];
+#pragma unroll
+            for (int j = 0; j < kBigU; j++) need |= k0 + 256 * j < ke && ((wd[j] >> (u[j] & 31)) & 1u);
+        }
+        if (!__syncthreads_or(need)) continue;
+        const bool owner1 = v >= g.n0;
+        long long best = owner1 ? LLONG_MAX : LLONG_MIN;
+        bool out = false;
+        for (int k0 = kb + (int)threadIdx.x; k0 < ke; k0 += kBigU * 256) {
+            int2 ew[kBigU];
+            long long hu[kBigU];
+#pragma unroll
+            for (int j = 0; j < kBigU; j++) ew[j] = k0 + 256 * j < ke ? g.edges[k0 + 256 * j] : make_int2(0, 0);
+#pragma unroll
+            for (int j = 0; j < kBigU; j++) hu[j] = h[ew[j].x];
+#pragma unroll
+            for (int j = 0; j < kBigU; j++) {
+                if (k0 + 256 * j >= ke) break;
+                if (hu[j] == kHOut) { out = true; continue; }
+                const long long c = hu[j] == kHTop ? LLONG_MIN : scale * ew[j].y + 1 + hu[j];
+                best = owner1 ? min(best, c) : max(best, c);
+            }
+        }
+        out = __syncthreads_or(out);
+        best = egs_block_minmax_ll(best, owner1, sm);
+        if (threadIdx.x == 0) {
+            if (owner1 ? best == LLONG_MAX : out) {
+                h[v] = kHOut;
+                atomicOr(&cur[v >> 5], 1u << (v & 31));
+                local++;
+            } else if (best > hv) {
+                h[v] = best;
+                atomicOr(&cur[v >> 5], 1u << (v & 31));
+                local++;
+                if (best > bound) atomicOr(&flags[1], 1);
+            }
+        }
+    }
+    if (threadIdx.x == 0 && local) atomicAdd(&cnt[p % kCntSlots], local);
 }
 
 // D is where h >= 0 (set by k_closure for this candidate).
@@ -268,11 +418,13 @@ int gap_candidates(const int *win, const int *flo, int wmax, int mg, int64_t *ou
 
 // One certification attempt on a paused run after round `last`.  Candidates, in turn:
 // every admissible gap threshold (smallest first, at most max_cands), then the vertices
-// that rose in round `last`.  Each gets the closure check and at most `cap` passes of the
-// longest-path game; the first that converges is set to TOP.  Returns 1 when vertices were
+// that rose in round `last`.  Each is pruned to its greatest closed subset while at most
+// `cap` passes of the longest-path game run on it; the first that converges with a
+// non-empty set is set to TOP.  has_big: some vertex is above big_deg (its own kernels).  Returns 1 when vertices were
 // set to TOP (count in res.marked), 0 when no candidate gave a proof, < 0 on a HIP error.
-int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int cap, int max_cands,
+int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int cap, int max_cands, bool has_big,
                 hipStream_t st, CertResult &res) {
+    constexpr int kBigGrid = 512;           // workgroups of the kernels for vertices above big_deg
     res = CertResult{};
     if (wmax <= 0 || !g.rise) return 0;
     const long long scale = (long long)g.n + 1;
@@ -308,20 +460,26 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
         if (hipMemsetAsync(cb.bfcnt, 0, sizeof(int) * kCntSlots, st) != hipSuccess) return -1;
         hipLaunchKernelGGL(k_closure, dim3(grid), dim3(256), 0, st, g, B, last, bound, cb.h, bm[0], bm[1], cb.bfcnt,
                            flags);
+        if (has_big)
+            hipLaunchKernelGGL(k_closure_big, dim3(kBigGrid), dim3(256), 0, st, g, B, last, bound, cb.h, bm[0], cb.bfcnt,
+                               flags);
         int queued = 1;                     // passes queued so far: pass 0 is the closure's
         int passes = 0;
         bool converged = false, diverged = false, rejected = false;
         while (!converged && !diverged && !rejected && passes < cap) {
             const int upto = std::min(cap, queued == 1 ? first_chunk : queued + chunk);
-            for (int p = queued; p < upto; p++)
+            for (int p = queued; p < upto; p++) {
                 launch_bf_pass(G, bf_grid, st, g, cb.h, bound, bm[(p - 1) % 3], bm[p % 3], bm[(p + 1) % 3], p, cb.bfcnt,
                                flags);
+                if (has_big)
+                    hipLaunchKernelGGL(k_bf_big, dim3(kBigGrid), dim3(256), 0, st, g, cb.h, bound, bm[(p - 1) % 3],
+                                       bm[p % 3], p, cb.bfcnt, flags);
+            }
             if (hipMemcpyAsync(fl, flags, sizeof(int) * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
             if (hipMemcpyAsync(cnt, cb.bfcnt, sizeof(int) * kCntSlots, hipMemcpyDeviceToHost, st) != hipSuccess)
                 return -1;
             if (hipStreamSynchronize(st) != hipSuccess) return -1;
             if (fl[3] == 0) { rejected = true; break; }
-            if (fl[0]) { res.closure_failed++; rejected = true; break; }
             if (fl[1]) diverged = true;
             for (int p = queued - (queued > 1 ? 0 : 1); p < upto && !diverged; p++) {
                 passes = p + 1;
@@ -330,6 +488,7 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
             queued = upto;
         }
         if (rejected) continue;
+        if (fl[0]) res.closure_failed++;
         res.passes += passes;
         if (!converged) {
             if (!diverged) res.cap_hit = 1;
@@ -339,7 +498,7 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
         if (hipMemcpyAsync(fl + 2, flags + 2, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
         if (hipStreamSynchronize(st) != hipSuccess) return -1;
         res.marked = fl[2];
-        return res.marked > 0 ? 1 : 0;
+        if (res.marked > 0) return 1;       // else pruning emptied the candidate: try the next
     }
     return 0;
 }

@@ -523,10 +523,11 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
             egs::CertBuffers cb{s->d_hist, s->d_h, s->d_bfcnt, s->d_bits, s->h_probe + kPinCert};
             egs::CertResult cr;
             const int got = egs::certify_top(s->g, cb, s->wmax, (int)(s->next_round - 1), s->cert_cap,
-                                             kCertCandidates, s->stream, cr);
+                                             kCertCandidates, s->sh.use_wave != 0, s->stream, cr);
             if (got < 0) return fail(EGS_E_HIP, "certification: %s", hipGetErrorString(hipGetLastError()));
             s->st.cert_attempts++;
             s->st.cert_passes += cr.passes;
+            s->st.cert_pruned += cr.closure_failed;
             if (got > 0) {
                 s->st.certificates++;
                 s->st.top_marked += cr.marked;

@@ -77,3 +77,25 @@ __device__ __forceinline__ void egs_append(int *list, int *count, bool want, int
     base = __builtin_amdgcn_readfirstlane(base);
     if (want) list[base + __popcll(mask & egs_lanemask_lt())] = item;
 }
+
+// ---- workgroup-per-vertex helpers (vertices above big_deg) -----------------------------
+// The static lists of big (big_deg < deg <= huge_deg) and huge (deg > huge_deg) vertices,
+// read as one sequence of nb + nh items.
+__device__ __forceinline__ int egs_big_item(const int *sbig, const int *shuge, int nb, int64_t i) {
+    return i < nb ? sbig[i] : shuge[i - nb];
+}
+
+// Workgroup-wide min (take_min) or max of a 64-bit value, returned to every thread.  `sm`
+// holds one slot per wave; every thread of the workgroup must call.
+__device__ __forceinline__ long long egs_block_minmax_ll(long long x, bool take_min, long long *sm) {
+    for (int d = 32; d >= 1; d >>= 1) {
+        const long long o = __shfl_xor(x, d, 64);
+        x = take_min ? (o < x ? o : x) : (o > x ? o : x);
+    }
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = x;
+    __syncthreads();
+    long long r = sm[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = take_min ? (sm[w] < r ? sm[w] : r) : (sm[w] > r ? sm[w] : r);
+    __syncthreads();
+    return r;
+}

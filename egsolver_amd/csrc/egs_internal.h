@@ -97,12 +97,12 @@ struct CertResult {
     int tried;           // candidates tried
     int passes;          // longest-path passes over all candidates
     int marked;          // vertices set to TOP
-    int closure_failed;  // candidates rejected by the closure check
+    int closure_failed;  // candidates that lost vertices to the closure check
     int cap_hit;         // some candidate ran out of passes (neither converged nor diverged)
 };
 hipError_t launch_abs_wmax(const DevGame &g, int *out, hipStream_t st);
 int gap_candidates(const int *win, const int *flo, int wmax, int mg, int64_t *out, int max_out);
-int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int cap, int max_cands,
+int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int cap, int max_cands, bool has_big,
                 hipStream_t st, CertResult &res);
 
 }  // namespace egs

@@ -186,12 +186,37 @@ __device__ __forceinline__ void flush_stats(const DevGame &g, LiftAcc &a) {
 
 // min/max over v's out-edges [s, e) of OMINUS(val[succ], w), lanes of a group of G
 // striding; returns the group result in every lane of the group.
+// Min-key over the edges [s, e) taken by lane l of `stride` lanes, U edges in flight per
+// lane: the vertices given to the wavefront and workgroup kernels have hundreds to tens of
+// thousands of edges, and one gather at a time leaves that loop latency-bound.  Slots past
+// e load vertex 0's value and are not used.
+template <int U>
+__device__ __forceinline__ int key_over_edges(const DevGame &g, int s, int e, int l, int stride, bool owner1) {
+    int key = EGS_TOP;
+    for (int k0 = s + l; k0 < e; k0 += stride * U) {
+        int2 ew[U];
+        int a[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) ew[j] = k0 + j * stride < e ? g.edges[k0 + j * stride] : make_int2(0, 0);
+#pragma unroll
+        for (int j = 0; j < U; j++) a[j] = g.val[ew[j].x];
+#pragma unroll
+        for (int j = 0; j < U; j++)
+            if (k0 + j * stride < e) key = min(key, egs_key(egs_ominus(a[j], ew[j].y, g.mg), owner1));
+    }
+    return key;
+}
+
 template <int G>
 __device__ __forceinline__ int lift_value(const DevGame &g, int s, int e, int gl, bool owner1) {
     int key = EGS_TOP;
-    for (int k = s + gl; k < e; k += G) {
-        const int2 ew = g.edges[k];
-        key = min(key, egs_key(egs_ominus(g.val[ew.x], ew.y, g.mg), owner1));
+    if constexpr (G == 64) {
+        key = key_over_edges<4>(g, s, e, gl, 64, owner1);
+    } else {
+        for (int k = s + gl; k < e; k += G) {
+            const int2 ew = g.edges[k];
+            key = min(key, egs_key(egs_ominus(g.val[ew.x], ew.y, g.mg), owner1));
+        }
     }
     key = egs_group_min<G>(key);
     return egs_unkey(key, owner1);
@@ -244,12 +269,14 @@ __global__ __launch_bounds__(256) void k_init(DevGame g) {
         if (v < g.n) {
             const int s = g.ptr[v], e = g.ptr[v + 1];
             bool all_neg = true, any_neg = false;
-            for (int k = s; k < e; k++) {
-                const bool neg = g.edges[k].y < 0;
-                all_neg &= neg;
-                any_neg |= neg;
+            if (e - s <= g.big_deg) {             // above big_deg: k_init_big
+                for (int k = s; k < e; k++) {
+                    const bool neg = g.edges[k].y < 0;
+                    all_neg &= neg;
+                    any_neg |= neg;
+                }
+                act = (v < g.n0) ? all_neg : any_neg;
             }
-            act = (v < g.n0) ? all_neg : any_neg;
             g.val[v] = 0;
             g.stamp[v] = act ? 1 : 0;
             if (g.rise) g.rise[v] = 0;
@@ -258,6 +285,29 @@ __global__ __launch_bounds__(256) void k_init(DevGame g) {
         stage_flush(g, st, kLdsCap / 2);
     }
     stage_flush(g, st, 1);
+}
+
+// The vertices above big_deg, one workgroup each, after k_init: the same activity test
+// (owner 0: every edge negative, owner 1: some edge negative) and the same append.
+__global__ __launch_bounds__(256) void k_init_big(DevGame g) {
+    const int nb = g.ring[9], items = nb + g.ring[10];
+    for (int64_t i = blockIdx.x; i < items; i += gridDim.x) {
+        const int v = egs_big_item(g.sbig, g.shuge, nb, i);
+        const int s = g.ptr[v], e = g.ptr[v + 1];
+        bool all_neg = true, any_neg = false;
+        for (int k = s + (int)threadIdx.x; k < e; k += blockDim.x) {
+            const bool neg = g.edges[k].y < 0;
+            all_neg &= neg;
+            any_neg |= neg;
+        }
+        all_neg = __syncthreads_and(all_neg);
+        any_neg = __syncthreads_or(any_neg);
+        const bool act = (v < g.n0) ? all_neg : any_neg;
+        if (threadIdx.x < 64) {
+            if (threadIdx.x == 0 && act) g.stamp[v] = 1;
+            push_wave(g, g.list[1], g.lcnt + 1 * kListCounters, (int)(blockIdx.x % kSub), act && threadIdx.x == 0, v);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -600,11 +650,7 @@ __global__ __launch_bounds__(256) void k_lift_block(DevGame g, int r) {
         const int s = g.ptr[v], e = g.ptr[v + 1];
         const int old = g.val[v];
         const bool owner1 = v >= g.n0;
-        int key = EGS_TOP;
-        for (int k = s + (int)threadIdx.x; k < e; k += 256) {
-            const int2 ew = g.edges[k];
-            key = min(key, egs_key(egs_ominus(g.val[ew.x], ew.y, g.mg), owner1));
-        }
+        int key = key_over_edges<8>(g, s, e, (int)threadIdx.x, 256, owner1);
         key = egs_group_min<64>(key);
         if (lane == 0) wmin[wid] = key;
         __syncthreads();
@@ -635,6 +681,7 @@ hipError_t launch_init(const DevGame &g, hipStream_t st) {
     if (err == hipSuccess) err = hipMemsetAsync(g.ring, 0, sizeof(int) * 9, st);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_init, dim3(grid_for(g.n, 256, 2048)), dim3(256), 0, st, g);
+    hipLaunchKernelGGL(k_init_big, dim3(256), dim3(256), 0, st, g);
     return hipGetLastError();
 }
 

@@ -176,18 +176,40 @@ __global__ __launch_bounds__(256) void k_classify(const int *__restrict__ ptr, i
     }
 }
 
-// Fixpoint certificate: count vertices with lift(val)(v) != val(v).
+// Fixpoint certificate: count vertices with lift(val)(v) != val(v).  A lane takes its own
+// vertex when the out-degree is at most kViolLane; the wave then takes the larger ones
+// together, one at a time, so no lane walks a hub's edge list alone.
+constexpr int kViolLane = 64;
+
 __global__ __launch_bounds__(256) void k_violations(DevGame g, unsigned long long *out) {
     unsigned long long bad = 0;
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n;
-         v += (int64_t)gridDim.x * blockDim.x) {
-        const bool owner1 = v >= g.n0;
-        int key = EGS_TOP;
-        for (int k = g.ptr[v]; k < g.ptr[v + 1]; k++) {
-            const int2 ew = g.edges[k];
-            key = min(key, egs_key(egs_ominus(g.val[ew.x], ew.y, g.mg), owner1));
+    const int lane = threadIdx.x & 63;
+    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x; v0 < g.n; v0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = v0 + threadIdx.x;
+        int s = 0, e = 0;
+        if (v < g.n) { s = g.ptr[v]; e = g.ptr[v + 1]; }
+        const bool wide = e - s > kViolLane;
+        if (v < g.n && !wide) {
+            const bool owner1 = v >= g.n0;
+            int key = EGS_TOP;
+            for (int k = s; k < e; k++) {
+                const int2 ew = g.edges[k];
+                key = min(key, egs_key(egs_ominus(g.val[ew.x], ew.y, g.mg), owner1));
+            }
+            bad += egs_unkey(key, owner1) != g.val[v];
         }
-        bad += egs_unkey(key, owner1) != g.val[v];
+        for (uint64_t mask = __ballot(wide); mask; mask &= mask - 1) {
+            const int b = __ffsll((unsigned long long)mask) - 1;
+            const int vb = __shfl((int)v, b, 64), sb = __shfl(s, b, 64), eb = __shfl(e, b, 64);
+            const bool owner1 = vb >= g.n0;
+            int key = EGS_TOP;
+            for (int k = sb + lane; k < eb; k += 64) {
+                const int2 ew = g.edges[k];
+                key = min(key, egs_key(egs_ominus(g.val[ew.x], ew.y, g.mg), owner1));
+            }
+            key = egs_group_min<64>(key);
+            if (lane == 0) bad += egs_unkey(key, owner1) != g.val[vb];
+        }
     }
     for (int d = 32; d >= 1; d >>= 1) bad += __shfl_xor(bad, d, 64);
     if ((threadIdx.x & 63) == 0 && bad) atomicAdd(out, bad);

@@ -77,6 +77,7 @@ typedef struct egs_stats {
     int64_t certificates;   /* TOP proofs that succeeded */
     int64_t cert_passes;    /* longest-path passes spent on proofs */
     int64_t top_marked;     /* vertices set to TOP by proofs */
+    int64_t cert_pruned;    /* proof candidates that lost vertices to the closure check */
 } egs_stats;
 
 /* ---- device ---------------------------------------------------------------------- */

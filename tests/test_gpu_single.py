@@ -165,6 +165,53 @@ def test_phased_dense_rounds_vs_oracle(oracle, seed):
             assert s.violations() == 0
 
 
+def test_pruned_proofs_and_hubs_vs_oracle(oracle):
+    """Power-law games with hub vertices, proofs attempted every 4 rounds: candidates that fail
+    the closure check are pruned to their greatest closed subset before and during the
+    longest-path passes, and vertices above big_deg go to the workgroup-per-vertex kernels of
+    the rounds, the closure and the passes (forced down to degree 8 / 32 in one variant).
+    Exact against the oracle, and pruning must have happened."""
+    import egsolver_amd as E
+    pruned = certs = 0
+    for seed in range(6):
+        n = 3000 + 2500 * seed
+        p = E.GenParams(n=n, seed=900 + seed, deg_kind=2, deg_lo=1, deg_hi=n - 1, alpha=2.1, wmax=100)
+        g = E.generate(p)
+        _say(f"pruned proofs seed {seed} n {g.n} m {g.m} maxdeg {g.max_outdeg}")
+        ref = lfp(oracle, to_oracle_game(g))
+        with E.Solver(g.n, g.m) as s:
+            s.load(g)
+            for opt in (E.options(certify_every=4), E.options(certify_every=4, big_deg=8, huge_deg=32),
+                        E.options(certify_every=12, dense_pct=101)):
+                s.prepare(opt)
+                conv, st = s.iterate()
+                assert conv
+                assert np.array_equal(s.values(), ref), (seed, opt.big_deg, opt.certify_every)
+                assert s.violations() == 0
+                pruned += st.cert_pruned
+                certs += st.certificates
+    _say(f"pruned proofs: {certs} certificates, {pruned} pruned candidates")
+    assert certs > 0 and pruned > 0, (certs, pruned)
+
+
+def test_violations_count_with_hubs(oracle):
+    """The device fixpoint check (wide vertices shared by the wave) counts the same
+    violations as the oracle on the partial values of a power-law game."""
+    import egsolver_amd as E
+    g = E.generate(E.GenParams(n=20000, seed=77, deg_kind=2, deg_lo=1, deg_hi=19999, alpha=1.9, wmax=60))
+    _say(f"violations with hubs n {g.n} m {g.m} maxdeg {g.max_outdeg}")
+    og = to_oracle_game(g)
+    with E.Solver(g.n, g.m) as s:
+        s.load(g)
+        s.prepare(E.options(certify_every=-1))
+        for k in (1, 3, 9):
+            s.iterate(max_phases=k)
+            part = s.values()
+            want = oracle.fixpoint_violations(og, part)
+            assert want > 0
+            assert s.violations() == want, k
+
+
 def test_phased_equals_single_sweep_full_size():
     """Config 2 at full size: phased and single-sweep dense rounds end on the same values
     (no oracle finishes this size; the device fixpoint check covers it)."""
