@@ -81,11 +81,21 @@ struct egs_batch {
     bool uploaded = false;
     int lds_limit = 0, cus = 0;
     int packed_games = 0;
+    int *d_values = nullptr;     // every game's values, contiguous in game order
+    int64_t total_n = 0;
+    // read-back staging: two pinned chunks, filled by the device while the host copies
+    // the other one out
+    int32_t *h_stage[2] = {nullptr, nullptr};
+    hipEvent_t ev_stage[2] = {nullptr, nullptr};
 };
+
+constexpr size_t kStageInts = (size_t)2 << 20;   // 8 MB per read-back chunk
 
 static void batch_free_device(egs_batch *b) {
     if (b->d_block) (void)hipFree(b->d_block);
     b->d_block = nullptr;
+    b->d_values = nullptr;
+    b->total_n = 0;
     b->uploaded = false;
 }
 
@@ -106,7 +116,9 @@ int egs_batch_create(int device, egs_batch **out) {
     b->lds_limit = (int)p.sharedMemPerBlock;
     b->cus = p.multiProcessorCount;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&b->ev_a) != hipSuccess || hipEventCreate(&b->ev_b) != hipSuccess) {
+        hipEventCreate(&b->ev_a) != hipSuccess || hipEventCreate(&b->ev_b) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_stage[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_stage[1], hipEventDisableTiming) != hipSuccess) {
         delete b;
         return bfail(EGS_E_HIP, "stream/event creation failed");
     }
@@ -121,6 +133,10 @@ int egs_batch_destroy(egs_batch *b) {
     batch_free_device(b);
     if (b->ev_a) (void)hipEventDestroy(b->ev_a);
     if (b->ev_b) (void)hipEventDestroy(b->ev_b);
+    for (int k = 0; k < 2; k++) {
+        if (b->ev_stage[k]) (void)hipEventDestroy(b->ev_stage[k]);
+        if (b->h_stage[k]) (void)hipHostFree(b->h_stage[k]);
+    }
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return EGS_OK;
@@ -167,16 +183,20 @@ int egs_batch_upload(egs_batch *b) {
     batch_free_device(b);
     const size_t ng = b->games.size();
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    // per game: ptr, edges, values, rise stamps, h
+    // all values first (contiguous, game order: one copy reads them back), then per game:
+    // ptr, edges, rise stamps, h
     std::vector<size_t> at(ng);
     std::vector<int> shift(ng);
-    size_t off = al(sizeof(egs::GameDesc) * ng) + al(sizeof(egs::GameState) * ng) + al(4 * ng) + 256;
+    int64_t total_n = 0;
+    for (size_t i = 0; i < ng; i++) total_n += b->games[i].n;
+    const size_t o_values = al(sizeof(egs::GameDesc) * ng) + al(sizeof(egs::GameState) * ng) + al(4 * ng) + 256;
+    size_t off = o_values + al(4 * (size_t)total_n);
     for (size_t i = 0; i < ng; i++) {
         const Staged &g = b->games[i];
         shift[i] = egs::pack_shift(g.n, g.wmax);
         at[i] = off;
         off += al(4 * ((size_t)g.n + 1)) + al((shift[i] ? 4 : 8) * (size_t)g.m) + al(4 * (size_t)g.n) +
-               al(4 * (size_t)g.n) + al(8 * (size_t)g.n);
+               al(8 * (size_t)g.n);
     }
     if (hipMalloc(&b->d_block, off) != hipSuccess) return bfail(EGS_E_NOMEM, "hipMalloc of %zu bytes failed", off);
     char *base = (char *)b->d_block;
@@ -184,21 +204,24 @@ int egs_batch_upload(egs_batch *b) {
     b->d_states = (egs::GameState *)(base + al(sizeof(egs::GameDesc) * ng));
     b->d_order = (int *)(base + al(sizeof(egs::GameDesc) * ng) + al(sizeof(egs::GameState) * ng));
     b->d_counter = (int *)((char *)b->d_order + al(4 * ng));
+    b->d_values = (int *)(base + o_values);
+    b->total_n = total_n;
     b->descs.assign(ng, egs::GameDesc{});
     b->states.assign(ng, egs::GameState{});
     b->packed_games = 0;
     std::vector<uint32_t> packed;
     std::vector<int2> wide;
+    int64_t vat = 0;
     for (size_t i = 0; i < ng; i++) {
         const Staged &g = b->games[i];
         char *p = base + at[i];
         egs::GameDesc &d = b->descs[i];
+        d.values = b->d_values + vat;
+        vat += g.n;
         d.ptr = (const int *)p;
         p += al(4 * ((size_t)g.n + 1));
         d.edges = p;
         p += al((shift[i] ? 4 : 8) * (size_t)g.m);
-        d.values = (int *)p;
-        p += al(4 * (size_t)g.n);
         d.rise = (int *)p;
         p += al(4 * (size_t)g.n);
         d.h = (long long *)p;
@@ -306,12 +329,22 @@ int egs_batch_read_values(egs_batch *b, int game, int32_t *out) {
 int egs_batch_read_all(egs_batch *b, int32_t *out) {
     if (!b || !b->uploaded) return bfail(EGS_E_STATE, "nothing uploaded");
     BHIP(hipSetDevice(b->device));
-    size_t at = 0;
-    for (size_t i = 0; i < b->games.size(); i++) {
-        BHIP(hipMemcpyAsync(out + at, b->descs[i].values, 4 * (size_t)b->games[i].n, hipMemcpyDeviceToHost, b->stream));
-        at += b->games[i].n;
+    for (int k = 0; k < 2; k++)
+        if (!b->h_stage[k] && hipHostMalloc((void **)&b->h_stage[k], 4 * kStageInts, hipHostMallocDefault) != hipSuccess)
+            return bfail(EGS_E_NOMEM, "hipHostMalloc of the read-back staging failed");
+    const int64_t chunks = (b->total_n + (int64_t)kStageInts - 1) / (int64_t)kStageInts;
+    auto len = [&](int64_t c) { return (size_t)std::min<int64_t>((int64_t)kStageInts, b->total_n - c * (int64_t)kStageInts); };
+    for (int64_t c = 0; c <= chunks; c++) {
+        if (c < chunks) {   // the buffer's previous chunk (c - 2) was copied out in step c - 1
+            BHIP(hipMemcpyAsync(b->h_stage[c & 1], b->d_values + c * (int64_t)kStageInts, 4 * len(c),
+                                hipMemcpyDeviceToHost, b->stream));
+            BHIP(hipEventRecord(b->ev_stage[c & 1], b->stream));
+        }
+        if (c > 0) {
+            BHIP(hipEventSynchronize(b->ev_stage[(c - 1) & 1]));
+            egs_pk_copy(out + (c - 1) * (int64_t)kStageInts, b->h_stage[(c - 1) & 1], 4 * len(c - 1));
+        }
     }
-    BHIP(hipStreamSynchronize(b->stream));
     return EGS_OK;
 }
 
