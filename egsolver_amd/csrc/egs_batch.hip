@@ -196,7 +196,7 @@ int egs_batch_upload(egs_batch *b) {
         shift[i] = egs::pack_shift(g.n, g.wmax);
         at[i] = off;
         off += al(4 * ((size_t)g.n + 1)) + al((shift[i] ? 4 : 8) * (size_t)g.m) + al(4 * (size_t)g.n) +
-               al(8 * (size_t)g.n);
+               al(4 * (size_t)g.n);
     }
     if (hipMalloc(&b->d_block, off) != hipSuccess) return bfail(EGS_E_NOMEM, "hipMalloc of %zu bytes failed", off);
     char *base = (char *)b->d_block;
@@ -224,7 +224,7 @@ int egs_batch_upload(egs_batch *b) {
         p += al((shift[i] ? 4 : 8) * (size_t)g.m);
         d.rise = (int *)p;
         p += al(4 * (size_t)g.n);
-        d.h = (long long *)p;
+        d.h = (int *)p;
         d.state = b->d_states + i;
         d.m = g.m;
         d.n = g.n;

@@ -9,7 +9,8 @@
 // cursor and whether the sweep has changed anything so far, so a later launch resumes it.
 // A sweep that changed nothing proves the fixpoint even when it spanned launches: no value
 // moved while it ran, so every vertex was checked against the final values.
-// Values sit in LDS when the game fits the workgroup's LDS share, else in global memory.
+// The first values of a game (as many as the workgroup's LDS share holds) sit in LDS, the
+// rest in global memory.
 // Every `every` sweeps (doubling while proofs fail) the workgroup tries the certified TOP
 // proof of egs_accel.hip on its game.
 #pragma once
@@ -41,7 +42,7 @@ struct GameDesc {
     const void *edges;     // m edges: uint32 (w << shift) | succ, or int2 {succ, w} when shift == 0
     int *values;           // n
     int *rise;             // n: number of the sweep in which each vertex last rose (proof candidates)
-    long long *h;          // n scratch (proofs: longest-path potentials)
+    int *h;                // n scratch (proofs: potentials above the warm start)
     GameState *state;
     int64_t m;
     int n, n0, mg, wmax;
@@ -59,6 +60,23 @@ struct CertArgs {
     int candidates;        // thresholds tried per attempt
 };
 
+// A game's values: the first `split` in LDS (lo), the rest in the game's global array (hi,
+// indexed by vertex, so hi[v] for v >= split).  A game that fits the workgroup's LDS share
+// has split == n; a larger one keeps most of its gathers in LDS and the remainder in L2.
+struct GameVals {
+    int *lo;
+    int *hi;
+    int split;
+    __device__ __forceinline__ int get(int v) const { return v < split ? lo[v] : hi[v]; }
+    __device__ __forceinline__ void set(int v, int x) const {
+        if (v < split) lo[v] = x;
+        else hi[v] = x;
+    }
+};
+
+template <bool SPLIT>
+__device__ __forceinline__ int gv(const GameVals &V, int v) { return SPLIT ? V.get(v) : V.lo[v]; }
+
 __device__ __forceinline__ void game_edge(const GameDesc &b, int64_t k, int &succ, int &w) {
     if (b.shift) {
         const uint32_t e = ((const uint32_t *)b.edges)[k];
@@ -71,12 +89,17 @@ __device__ __forceinline__ void game_edge(const GameDesc &b, int64_t k, int &suc
     }
 }
 
+// Edge loads in flight per lane: a lane's edges are read one after another from global
+// memory, and one load at a time leaves the loops latency-bound.  Slots past a vertex's
+// last edge read vertex 0 and are not used.
+constexpr int kGameU = 4;
+
 // Part of a sweep by groups of G lanes, from vertex `cursor`, chunk by chunk (a chunk is
 // one vertex per group) while `spent` < `budget`; returns the next cursor.  `spent` adds
 // the chunk's edge count, read from the offsets by every lane, so it stays block-uniform
 // and the loop keeps waves whole for the DPP reductions.
-template <int G>
-__device__ __forceinline__ int game_sweep_part(const GameDesc &b, int *val, int cursor, int64_t budget, int sweep_no,
+template <int G, bool SPLIT>
+__device__ __forceinline__ int game_sweep_part(const GameDesc &b, const GameVals &V, int cursor, int64_t budget, int sweep_no,
                                                int64_t &spent, int64_t &edges_done, int &lifted, bool &changed) {
     const int gl = threadIdx.x % G;
     const int gid = threadIdx.x / G;
@@ -90,21 +113,31 @@ __device__ __forceinline__ int game_sweep_part(const GameDesc &b, int *val, int 
         if (valid) {
             s = b.ptr[v];
             e = b.ptr[v + 1];
-            old = val[v];
+            old = gv<SPLIT>(V, v);
         }
         const bool owner1 = v >= b.n0;
         int key = EGS_TOP;
-        for (int k = s + gl; k < e; k += G) {
-            int succ, w;
-            game_edge(b, k, succ, w);
-            key = min(key, egs_key(egs_ominus(val[succ], w, b.mg), owner1));
+        for (int k0 = s + gl; k0 < e; k0 += G * kGameU) {
+            int su[kGameU], wv[kGameU], a[kGameU];
+#pragma unroll
+            for (int j = 0; j < kGameU; j++) {
+                su[j] = 0;
+                wv[j] = 0;
+                if (k0 + j * G < e) game_edge(b, k0 + j * G, su[j], wv[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < kGameU; j++) a[j] = gv<SPLIT>(V, su[j]);
+#pragma unroll
+            for (int j = 0; j < kGameU; j++)
+                if (k0 + j * G < e) key = min(key, egs_key(egs_ominus(a[j], wv[j], b.mg), owner1));
         }
         key = egs_group_min<G>(key);
         const int t = egs_unkey(key, owner1);
         if (valid && gl == 0) {
             edges_done += e - s;
             if (old < t) {
-                val[v] = t;
+                if (SPLIT) V.set(v, t);
+                else V.lo[v] = t;
                 b.rise[v] = sweep_no;
                 changed = true;
                 lifted++;
@@ -116,11 +149,12 @@ __device__ __forceinline__ int game_sweep_part(const GameDesc &b, int *val, int 
 }
 
 // The certified TOP proof inside the workgroup (the steps of egs_accel.hip).  Candidate
-// sets D, tried in turn until one passes both checks:
+// sets D, tried in turn until one yields a non-empty proven set:
 //   * every admissible gap threshold B, smallest first: D = {v : B < f(v) < TOP};
 //   * the vertices that rose in the last sweep (`last`): D = {v : rise(v) == last, f(v) < TOP}.
 // Check 1 (closure): an owner-0 vertex of D has every successor in D or TOP, an owner-1
-// vertex at least one.  Check 2: the longest-path game on D with w' = (n+1) w + 1,
+// vertex at least one; vertices that break it are removed until it holds (the greatest
+// closed subset), during the passes of check 2: the longest-path game on D with w' = (n+1) w + 1,
 //   h(v) <- max(h(v), max_{u in D} w'(v,u) + h(u))           (owner 0)
 //   h(v) <- max(h(v), min_{u in D or TOP} w'(v,u) + h(u))    (owner 1; a TOP successor ends the path)
 // reaches a fixpoint within `cap` passes.  At a fixpoint the minimising successors form a
@@ -128,15 +162,16 @@ __device__ __forceinline__ int game_sweep_part(const GameDesc &b, int *val, int 
 // the proof; any fixpoint will do, so the iteration starts from h = (n+1) (MG - f): the
 // lifting keeps f(u) - f(v) >= w(v,u) along such edges, so this start is already close.
 // Returns the vertices set to TOP; `cap_hit` tells whether some candidate ran out of passes.
-constexpr long long kGameHOut = LLONG_MIN;       // potential of a vertex outside D
-constexpr long long kGameHTop = LLONG_MIN + 1;   // potential of a TOP vertex
+// Potentials are kept as d(v) = h(v) - (n+1) (MG - f(v)) >= 0, 4 bytes next to the values;
+// kGameDOut marks a vertex outside D (TOP is read off the value).  A d past INT_MAX ends the
+// attempt as if it diverged: it only gives up a proof.
+constexpr int kGameDOut = -1;
 
-__device__ __forceinline__ bool game_in_d(const GameDesc &b, const int *val, int v, int B, int last) {
-    const int x = val[v];
+__device__ __forceinline__ bool game_in_d_x(const GameDesc &b, int x, int v, int B, int last) {
     return x != EGS_TOP && (B >= 0 ? x > B : b.rise[v] == last);
 }
 
-__device__ int game_certify(const GameDesc &b, int *val, long long *hp, int last, int cap, int max_cands,
+__device__ int game_certify(const GameDesc &b, const GameVals &V, int *dp, int last, int cap, int max_cands,
                             int64_t &passes_done,
                             bool &cap_hit) {
     __shared__ int win[64], flo[64];
@@ -152,7 +187,7 @@ __device__ int game_certify(const GameDesc &b, int *val, long long *hp, int last
     for (int i = threadIdx.x; i < 64; i += blockDim.x) { win[i] = 0; flo[i] = 0; }
     __syncthreads();
     for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
-        const int x = val[v];
+        const int x = V.get(v);
         if (x == EGS_TOP || x <= 0) continue;
         const int64_t q = ((int64_t)x + wmax - 1) / wmax;
         if ((q & (q - 1)) == 0) atomicAdd(&win[63 - __clzll((unsigned long long)q)], 1);
@@ -190,74 +225,122 @@ __device__ int game_certify(const GameDesc &b, int *val, long long *hp, int last
             cand &= ~(1ull << j);
             B = (int)((int64_t)wmax << j);
         }
-        bool bad = false;
+        // pass 0: membership, closure and the first longest-path step, from the values alone
+        // (the warm start of a successor is (n+1) (MG - f)).  A member that breaks closure
+        // leaves D (h = kGameHOut); later passes check it again against the removals, so the
+        // passes compute the greatest closed subset of the candidate along with h.
+        bool ch = false, over = false;
         int members = 0;
         for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
-            // h >= 0 marks D for the passes; sentinels elsewhere (one 8-byte read per edge)
-            if (val[v] == EGS_TOP) { hp[v] = kGameHTop; continue; }
-            if (!game_in_d(b, val, v, B, last)) { hp[v] = kGameHOut; continue; }
+            const int x = V.get(v);
+            if (!game_in_d_x(b, x, v, B, last)) { dp[v] = kGameDOut; continue; }
             members++;
-            hp[v] = scale * ((long long)b.mg - val[v]);
-            const int s = b.ptr[v], e = b.ptr[v + 1];
+            const long long h0 = scale * ((long long)b.mg - x);
+            const bool owner1 = v >= b.n0;
+            long long best = owner1 ? LLONG_MAX : LLONG_MIN;
             bool any = false, all = true;
-            for (int k = s; k < e; k++) {
-                int u, w;
-                game_edge(b, k, u, w);
-                const bool ok = val[u] == EGS_TOP || game_in_d(b, val, u, B, last);
-                any |= ok;
-                all &= ok;
-            }
-            if (v >= b.n0 ? !any : !all) bad = true;
-        }
-        const int in_set = __syncthreads_count(members > 0);
-        if (__syncthreads_or(bad) || in_set == 0) continue;
-        bool converged = false, diverged = false;
-        for (int p = 0; p < cap; p++) {
-            bool ch = false, over = false;
-            for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
-                const long long hv = hp[v];
-                if (hv < 0) continue;
-                long long best = hv;
-                const int s = b.ptr[v], e = b.ptr[v + 1];
-                if (v >= b.n0) {
-                    long long mn = LLONG_MAX;
-                    for (int k = s; k < e; k++) {
-                        int u, w;
-                        game_edge(b, k, u, w);
-                        const long long hu = hp[u];
-                        if (hu == kGameHTop) { mn = LLONG_MIN; break; }
-                        if (hu != kGameHOut) mn = min(mn, scale * w + 1 + hu);
-                    }
-                    if (mn != LLONG_MAX && mn > best) best = mn;
-                } else {
-                    for (int k = s; k < e; k++) {
-                        int u, w;
-                        game_edge(b, k, u, w);
-                        const long long hu = hp[u];
-                        if (hu >= 0) best = max(best, scale * w + 1 + hu);
+            const int s = b.ptr[v], e = b.ptr[v + 1];
+            for (int k0 = s; k0 < e; k0 += kGameU) {
+                int uu[kGameU], ww[kGameU], xu[kGameU];
+#pragma unroll
+                for (int j = 0; j < kGameU; j++) {
+                    uu[j] = 0;
+                    ww[j] = 0;
+                    if (k0 + j < e) game_edge(b, k0 + j, uu[j], ww[j]);
+                }
+#pragma unroll
+                for (int j = 0; j < kGameU; j++) xu[j] = V.get(uu[j]);
+#pragma unroll
+                for (int j = 0; j < kGameU; j++) {
+                    if (k0 + j >= e) break;
+                    if (xu[j] == EGS_TOP) {
+                        any = true;
+                        if (owner1) best = LLONG_MIN;
+                    } else if (game_in_d_x(b, xu[j], uu[j], B, last)) {
+                        any = true;
+                        const long long c = scale * ww[j] + 1 + scale * ((long long)b.mg - xu[j]);
+                        best = owner1 ? min(best, c) : max(best, c);
+                    } else {
+                        all = false;
                     }
                 }
-                if (best > hv) {
-                    hp[v] = best;
+            }
+            if (owner1 ? !any : !all) {
+                dp[v] = kGameDOut;
+                ch = true;
+            } else if (best > h0) {
+                dp[v] = (int)min(best - h0, (long long)INT_MAX);
+                ch = true;
+                over |= best > bound || best - h0 >= INT_MAX;
+            } else {
+                dp[v] = 0;
+            }
+        }
+        if (__syncthreads_count(members > 0) == 0) continue;
+        passes_done++;
+        bool converged = __syncthreads_count(ch) == 0, diverged = __syncthreads_or(over) != 0;
+        for (int p = 1; p < cap && !converged && !diverged; p++) {
+            ch = false;
+            over = false;
+            for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
+                const int dv = dp[v];
+                if (dv < 0) continue;
+                const long long h0 = scale * ((long long)b.mg - V.get(v));
+                const long long hv = h0 + dv;
+                const bool owner1 = v >= b.n0;
+                long long best = owner1 ? LLONG_MAX : LLONG_MIN;
+                bool out = false;
+                const int s = b.ptr[v], e = b.ptr[v + 1];
+                for (int k0 = s; k0 < e; k0 += kGameU) {
+                    int uu[kGameU], ww[kGameU], xu[kGameU], du[kGameU];
+#pragma unroll
+                    for (int j = 0; j < kGameU; j++) {
+                        uu[j] = 0;
+                        ww[j] = 0;
+                        if (k0 + j < e) game_edge(b, k0 + j, uu[j], ww[j]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < kGameU; j++) {
+                        xu[j] = V.get(uu[j]);
+                        du[j] = dp[uu[j]];
+                    }
+#pragma unroll
+                    for (int j = 0; j < kGameU; j++) {
+                        if (k0 + j >= e) break;
+                        if (xu[j] == EGS_TOP) {       // ends the path: -infinity for the game
+                            if (owner1) best = LLONG_MIN;
+                            continue;
+                        }
+                        if (du[j] < 0) { out = true; continue; }
+                        const long long c = scale * ww[j] + 1 + scale * ((long long)b.mg - xu[j]) + du[j];
+                        best = owner1 ? min(best, c) : max(best, c);
+                    }
+                }
+                // owner 1 breaks closure when no edge reaches D or TOP, owner 0 when any leaves
+                if (owner1 ? best == LLONG_MAX : out) {
+                    dp[v] = kGameDOut;
                     ch = true;
-                    over |= best > bound;
+                } else if (best > hv) {
+                    dp[v] = (int)min(best - h0, (long long)INT_MAX);
+                    ch = true;
+                    over |= best > bound || best - h0 >= INT_MAX;
                 }
             }
             passes_done++;
-            const int raised = __syncthreads_count(ch);
-            if (__syncthreads_or(over)) { diverged = true; break; }
-            if (raised == 0) { converged = true; break; }
+            converged = __syncthreads_count(ch) == 0;
+            diverged = __syncthreads_or(over) != 0;
         }
-        if (!converged) {
+        if (diverged || !converged) {
             if (!diverged) cap_hit = true;
             continue;
         }
         int mine = 0;
         for (int v = threadIdx.x; v < b.n; v += blockDim.x)
-            if (hp[v] >= 0) { val[v] = EGS_TOP; mine++; }
+            if (dp[v] >= 0) { V.set(v, EGS_TOP); mine++; }
         if (mine) atomicAdd(&s_marked, mine);
         __syncthreads();
-        return s_marked;
+        if (s_marked > 0) return s_marked;
+        __syncthreads();                      // pruning emptied the candidate: try the next
     }
     return 0;
 }
@@ -269,7 +352,7 @@ __device__ int game_certify(const GameDesc &b, int *val, long long *hp, int last
 // and the complete sweeps to `sweeps` (both block-uniform); returns whether it stopped
 // because a proof attempt is due.
 template <int G>
-__device__ bool game_run(const GameDesc &b, int *val, GameState &sst, int64_t budget, int64_t max_sweeps,
+__device__ bool game_run(const GameDesc &b, const GameVals &V, GameState &sst, int64_t budget, int64_t max_sweeps,
                          const CertArgs &ca, int64_t &spent, int64_t &sweeps) {
     // the fields the loop changes, as scalars (a whole-struct copy ends up in scratch)
     int cursor = sst.cursor;
@@ -281,7 +364,10 @@ __device__ bool game_run(const GameDesc &b, int *val, GameState &sst, int64_t bu
     bool proof_due = false;
     while (!converged && used < budget && done < max_sweeps) {
         bool ch = false;
-        cursor = game_sweep_part<G>(b, val, cursor, budget, (int)(sweeps0 + done + 1), used, edges_done, lifted, ch);
+        if (V.split < b.n)
+            cursor = game_sweep_part<G, true>(b, V, cursor, budget, (int)(sweeps0 + done + 1), used, edges_done, lifted, ch);
+        else
+            cursor = game_sweep_part<G, false>(b, V, cursor, budget, (int)(sweeps0 + done + 1), used, edges_done, lifted, ch);
         dirty = __syncthreads_or(ch || dirty) != 0;
         if (cursor < b.n) break;                    // budget spent part-way through the sweep
         done++;
@@ -318,13 +404,13 @@ __device__ bool game_run(const GameDesc &b, int *val, GameState &sst, int64_t bu
 
 // One proof attempt at a sweep boundary; schedules the next one (spacing back to `every`
 // after a proof, doubled after a failure, pass cap doubled when a candidate ran out).
-// `hp` holds the potentials: LDS after the values when both fit, else the game's array.
-__device__ void game_prove(const GameDesc &b, int *val, long long *hp, GameState &sst, const CertArgs &ca) {
+// `dp` holds the potentials: LDS after the values when both fit, else the game's array.
+__device__ void game_prove(const GameDesc &b, const GameVals &V, int *dp, GameState &sst, const CertArgs &ca) {
     const int cap = sst.bf_cap;
     __syncthreads();
     int64_t passes = 0;
     bool cap_hit = false;
-    const int marked = game_certify(b, val, hp, (int)sst.sweeps, cap, ca.candidates, passes, cap_hit);
+    const int marked = game_certify(b, V, dp, (int)sst.sweeps, cap, ca.candidates, passes, cap_hit);
     const bool hit = __syncthreads_or(cap_hit) != 0;
     if (threadIdx.x == 0) {
         sst.cert_passes += passes;
@@ -357,11 +443,11 @@ static __global__ __launch_bounds__(1024) void k_games(const GameDesc *__restric
         __syncthreads();
         if (idx >= ngames) break;          // every workgroup leaves once the games run out
         const GameDesc b = games[order[idx]];
-        const bool in_lds = b.n <= lds_cap;
-        int *val = in_lds ? lds_val : b.values;
-        // proof potentials (8 bytes each) in LDS after the values when the game is small enough
-        const int h_at = (b.n + 1) & ~1;
-        long long *hp = in_lds && h_at + 2 * b.n <= lds_cap ? (long long *)(lds_val + h_at) : b.h;
+        // the first lds_cap values in LDS, the rest (if any) in the game's global array
+        const GameVals V{lds_val, b.values, min(b.n, lds_cap)};
+        // proof potentials (4 bytes each) in LDS after the values when the game is small enough
+        const int d_at = (b.n + 3) & ~3;
+        int *dp = d_at + b.n <= lds_cap ? lds_val + d_at : b.h;
         if (threadIdx.x == 0) {
             s_st = *b.state;
             if (!s_st.started) {
@@ -378,11 +464,11 @@ static __global__ __launch_bounds__(1024) void k_games(const GameDesc *__restric
         const bool fresh = s_idx < 0;
         if (fresh) {
             for (int v = threadIdx.x; v < b.n; v += blockDim.x) {
-                val[v] = 0;
+                V.set(v, 0);
                 b.rise[v] = 0;
             }
-        } else if (in_lds) {
-            for (int v = threadIdx.x; v < b.n; v += blockDim.x) lds_val[v] = b.values[v];
+        } else {
+            for (int v = threadIdx.x; v < V.split; v += blockDim.x) lds_val[v] = b.values[v];
         }
         __syncthreads();
         int64_t sweeps_left = max_sweeps;
@@ -390,23 +476,21 @@ static __global__ __launch_bounds__(1024) void k_games(const GameDesc *__restric
             int64_t spent = 0, sweeps = 0;
             bool due;
             switch (b.group) {
-            case 1: due = game_run<1>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
-            case 2: due = game_run<2>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
-            case 4: due = game_run<4>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
-            case 8: due = game_run<8>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
-            case 16: due = game_run<16>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
-            case 32: due = game_run<32>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
-            default: due = game_run<64>(b, val, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            case 1: due = game_run<1>(b, V, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            case 2: due = game_run<2>(b, V, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            case 4: due = game_run<4>(b, V, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            case 8: due = game_run<8>(b, V, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            case 16: due = game_run<16>(b, V, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            case 32: due = game_run<32>(b, V, s_st, left, sweeps_left, ca, spent, sweeps); break;
+            default: due = game_run<64>(b, V, s_st, left, sweeps_left, ca, spent, sweeps); break;
             }
             left -= spent;
             sweeps_left -= sweeps;
             if (!due) break;
-            game_prove(b, val, hp, s_st, ca);
+            game_prove(b, V, dp, s_st, ca);
             if (left <= 0 || sweeps_left <= 0) break;
         }
-        if (in_lds) {
-            for (int v = threadIdx.x; v < b.n; v += blockDim.x) b.values[v] = lds_val[v];
-        }
+        for (int v = threadIdx.x; v < V.split; v += blockDim.x) b.values[v] = lds_val[v];
         if (threadIdx.x == 0) *b.state = s_st;
         __syncthreads();
     }

@@ -127,7 +127,7 @@ constexpr int kSliceKedges = 2048;   // default edge budget per workgroup per sl
 size_t footprint(int n, int64_t m, int wmax) {
     const int shift = egs::pack_shift(n, wmax);
     return al256(4 * ((size_t)n + 1)) + al256((shift ? 4 : 8) * (size_t)m) + al256(4 * (size_t)n) +
-           al256(4 * (size_t)n) + al256(8 * (size_t)n);
+           al256(4 * (size_t)n) + al256(4 * (size_t)n);
 }
 
 // Arena bytes of a large-route game: offsets, edges (32-bit packed when `packed`, else
@@ -536,7 +536,7 @@ int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t m
     d.edges = base + al256(ptr_b);
     d.values = (int *)(base + al256(ptr_b) + al256(edge_b));
     d.rise = (int *)((char *)d.values + al256(4 * (size_t)n));
-    d.h = (long long *)((char *)d.rise + al256(4 * (size_t)n));
+    d.h = (int *)((char *)d.rise + al256(4 * (size_t)n));
     d.state = s->d_states + ticket;
     d.m = m;
     d.n = n;
