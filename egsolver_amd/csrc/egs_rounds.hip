@@ -545,6 +545,17 @@ __global__ __launch_bounds__(256) void k_phase(DevGame g, int r, int s) {
 // Slice layout of the low-degree vertices' edges: counts per (slice, vertex) into sptr,
 // an in-place exclusive scan, then each vertex places its edges (no atomics).
 // Also marks the vertices above big_deg in bigmask (one bit per vertex, from a wave ballot).
+// Slice of a successor: phases of slice_len successors each (comparisons, no division).
+__device__ __forceinline__ int slice_of(int u, int slice_len) {
+    int q = 0;
+#pragma unroll
+    for (int b = 1; b < kPhases; b++) q += u >= b * slice_len;
+    return q;
+}
+
+// Edges read ahead per thread in the slice set-up kernels (one thread per vertex).
+constexpr int kSliceU = 4;
+
 __global__ __launch_bounds__(256) void k_slice_count(DevGame g, int *__restrict__ cnt,
                                                      unsigned long long *__restrict__ bigmask) {
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < g.n; base += (int64_t)gridDim.x * blockDim.x) {
@@ -555,10 +566,16 @@ __global__ __launch_bounds__(256) void k_slice_count(DevGame g, int *__restrict_
             const int s = g.ptr[v], e = g.ptr[v + 1];
             big = e - s > g.big_deg;
             if (!big) {
-                for (int k = s; k < e; k++) {
-                    const int sl = g.edges[k].x / g.slice_len;
+                for (int k0 = s; k0 < e; k0 += kSliceU) {
+                    int u[kSliceU];
 #pragma unroll
-                    for (int q = 0; q < kPhases; q++) c[q] += sl == q;
+                    for (int j = 0; j < kSliceU; j++) u[j] = k0 + j < e ? g.edges[k0 + j].x : -1;
+#pragma unroll
+                    for (int j = 0; j < kSliceU; j++) {
+                        const int sl = u[j] < 0 ? -1 : slice_of(u[j], g.slice_len);
+#pragma unroll
+                        for (int q = 0; q < kPhases; q++) c[q] += sl == q;
+                    }
                 }
             }
 #pragma unroll
@@ -576,14 +593,20 @@ __global__ __launch_bounds__(256) void k_slice_scatter(DevGame g, void *__restri
         int pos[kPhases];
 #pragma unroll
         for (int q = 0; q < kPhases; q++) pos[q] = g.sptr[(size_t)q * g.n + v];
-        for (int k = s; k < e; k++) {
-            const int2 ew = g.edges[k];
-            const int sl = ew.x / g.slice_len;
+        for (int k0 = s; k0 < e; k0 += kSliceU) {
+            int2 ew[kSliceU];
 #pragma unroll
-            for (int q = 0; q < kPhases; q++) {
-                if (sl != q) continue;
-                if (g.sshift) ((uint32_t *)sedges)[pos[q]++] = ((uint32_t)ew.y << g.sshift) | (uint32_t)ew.x;
-                else ((int2 *)sedges)[pos[q]++] = ew;
+            for (int j = 0; j < kSliceU; j++) ew[j] = k0 + j < e ? g.edges[k0 + j] : make_int2(-1, 0);
+#pragma unroll
+            for (int j = 0; j < kSliceU; j++) {
+                if (ew[j].x < 0) break;
+                const int sl = slice_of(ew[j].x, g.slice_len);
+#pragma unroll
+                for (int q = 0; q < kPhases; q++) {
+                    if (sl != q) continue;
+                    if (g.sshift) ((uint32_t *)sedges)[pos[q]++] = ((uint32_t)ew[j].y << g.sshift) | (uint32_t)ew[j].x;
+                    else ((int2 *)sedges)[pos[q]++] = ew[j];
+                }
             }
         }
     }
