@@ -69,10 +69,40 @@ __device__ __forceinline__ int wave_sum(int x) {
     return x;
 }
 
+// Workgroup sum of x added to *dst with one atomic per workgroup (one per wave on a single
+// address serialises in L2 when thousands of waves finish together).  Every thread calls.
+__device__ __forceinline__ void block_add(int x, int *dst) {
+    __shared__ int part[16];
+    x = wave_sum(x);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[w];
+        if (t) atomicAdd(dst, t);
+    }
+    __syncthreads();
+}
+
 // Potentials carry D membership for the passes: h >= 0 on D, and two sentinels elsewhere,
 // so a pass gathers one 8-byte word per edge.
 constexpr long long kHOut = LLONG_MIN;        // not in D, not TOP
 constexpr long long kHTop = LLONG_MIN + 1;    // TOP: a successor that ends the path
+
+// Changed-vertex bitmaps have two levels: ceil(n/32) words with a bit per vertex, then a
+// summary with a bit per word (set when the word may be non-zero).  A test reads the
+// summary first: it is small enough to stay in the L1 caches, and when few vertices
+// changed almost every test ends there.
+__device__ __forceinline__ int64_t bm_words(int n) { return ((int64_t)n + 31) >> 5; }
+__device__ __forceinline__ int64_t bm_total(int n) { return bm_words(n) + ((bm_words(n) + 31) >> 5); }
+__device__ __forceinline__ void bm_set(unsigned *bm, int n, int v) {
+    atomicOr(&bm[v >> 5], 1u << (v & 31));
+    atomicOr(&bm[bm_words(n) + (v >> 10)], 1u << ((v >> 5) & 31));
+}
+__device__ __forceinline__ bool bm_test(const unsigned *bm, int n, int u) {
+    if (!((bm[bm_words(n) + (u >> 10)] >> ((u >> 5) & 31)) & 1u)) return false;
+    return (bm[u >> 5] >> (u & 31)) & 1u;
+}
 
 // Pass counters live in a ring of kCntSlots ints: pass p adds its raises to slot p % kCntSlots
 // and zeroes slot (p + 1) % kCntSlots for the pass after it.
@@ -86,7 +116,8 @@ constexpr int kCntSlots = 64;
 // no potentials: the warm start (n+1) (MG - f) of a successor follows from its value, so it
 // is computed here from f (a Jacobi pass from the warm start).  Writes h for every vertex
 // (the first pass's result on D, else a sentinel), the changed vertices' bits into `bits`
-// (every word of the vertex range is stored), zeroes `clear` (the bitmap of the next pass),
+// (every word of the vertex range is stored; the summary must be zero on entry), zeroes
+// `clear` (the bitmap of the next pass),
 // counts changes into cnt[0], members into flags[3] and removals into flags[0].
 __global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, long long bound,
                                                  long long *__restrict__ h, unsigned *__restrict__ bits,
@@ -144,16 +175,18 @@ __global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, lon
         const unsigned long long bal = __ballot(up);
         if (lane == 0 || lane == 32) {
             const int64_t word = v >> 5;
-            bits[word] = (unsigned)(bal >> lane);
+            const unsigned bw = (unsigned)(bal >> lane);
+            bits[word] = bw;
+            if (bw) atomicOr(&bits[bm_words(g.n) + (word >> 5)], 1u << (word & 31));
             clear[word] = 0u;
         }
     }
-    members = wave_sum(members);
-    changed = wave_sum(changed);
-    removed = wave_sum(removed);
-    if (lane == 0 && members) atomicAdd(&flags[3], members);
-    if (lane == 0 && removed) atomicAdd(&flags[0], removed);
-    if (lane == 0 && changed) atomicAdd(&cnt[0], changed);
+    for (int64_t i = bm_words(g.n) + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < bm_total(g.n);
+         i += (int64_t)gridDim.x * blockDim.x)
+        clear[i] = 0u;
+    block_add(members, &flags[3]);
+    block_add(removed, &flags[0]);
+    block_add(changed, &cnt[0]);
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[1] = 0;
 }
 
@@ -205,12 +238,12 @@ __global__ __launch_bounds__(256) void k_closure_big(DevGame g, int B, int last,
             const long long h0 = scale * ((long long)g.mg - x);
             if (owner1 ? !any : !all) {
                 h[v] = kHOut;
-                atomicOr(&bits[v >> 5], 1u << (v & 31));
+                bm_set(bits, g.n, v);
                 atomicAdd(&cnt[0], 1);
                 atomicAdd(&flags[0], 1);
             } else if (best > h0) {
                 h[v] = best;
-                atomicOr(&bits[v >> 5], 1u << (v & 31));
+                bm_set(bits, g.n, v);
                 atomicAdd(&cnt[0], 1);
                 if (best > bound) atomicOr(&flags[1], 1);
             }
@@ -234,7 +267,7 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
                                                  unsigned *__restrict__ clear, int p, int *__restrict__ cnt,
                                                  int *__restrict__ flags) {
     if (cnt[(p - 1) % kCntSlots] == 0 || flags[1]) return;
-    const int64_t words = ((int64_t)g.n + 31) >> 5;
+    const int64_t words = bm_total(g.n);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
         clear[i] = 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[(p + 1) % kCntSlots] = 0;
@@ -255,10 +288,7 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
             kb = g.ptr[v];
             ke = g.ptr[v + 1];
             if (ke - kb > g.big_deg) ke = kb;         // k_bf_big evaluates it: no need here
-            for (int k = kb + gl; k < ke && !need; k += G) {
-                const int u = g.edges[k].x;
-                need = (prev[u >> 5] >> (u & 31)) & 1u;
-            }
+            for (int k = kb + gl; k < ke && !need; k += G) need = bm_test(prev, g.n, g.edges[k].x);
         }
         need = ((__ballot(need) >> gbase) & kGroupMask) != 0;
         long long best = owner1 ? LLONG_MAX : LLONG_MIN;
@@ -282,18 +312,17 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
             // owner 1 breaks closure when no edge reaches D or TOP, owner 0 when any leaves
             if (owner1 ? best == LLONG_MAX : out) {
                 h[v] = kHOut;
-                atomicOr(&cur[v >> 5], 1u << (v & 31));
+                bm_set(cur, g.n, v);
                 local++;
             } else if (best > hv) {
                 h[v] = best;
-                atomicOr(&cur[v >> 5], 1u << (v & 31));
+                bm_set(cur, g.n, v);
                 local++;
                 if (best > bound) atomicOr(&flags[1], 1);
             }
         }
     }
-    local = wave_sum(local);
-    if ((threadIdx.x & 63) == 0 && local) atomicAdd(&cnt[p % kCntSlots], local);
+    block_add(local, &cnt[p % kCntSlots]);
 }
 
 // Pass p for the vertices above big_deg (launched after k_bf_pass for the same p), one
@@ -318,9 +347,10 @@ __global__ __launch_bounds__(256) void k_bf_big(DevGame g, long long *__restrict
 #pragma unroll
             for (int j = 0; j < kBigU; j++) u[j] = k0 + 256 * j < ke ? g.edges[k0 + 256 * j].x : 0;
 #pragma unroll
-            for (int j = 0; j < kBigU; j++) wd[j] = prev[u[j] >> 5];
+            for (int j = 0; j < kBigU; j++) wd[j] = prev[bm_words(g.n) + (u[j] >> 10)];
 #pragma unroll
-            for (int j = 0; j < kBigU; j++) need |= k0 + 256 * j < ke && ((wd[j] >> (u[j] & 31)) & 1u);
+            for (int j = 0; j < kBigU; j++)
+                need |= k0 + 256 * j < ke && ((wd[j] >> ((u[j] >> 5) & 31)) & 1u) && ((prev[u[j] >> 5] >> (u[j] & 31)) & 1u);
         }
         if (!__syncthreads_or(need)) continue;
         const bool owner1 = v >= g.n0;
@@ -346,11 +376,11 @@ __global__ __launch_bounds__(256) void k_bf_big(DevGame g, long long *__restrict
         if (threadIdx.x == 0) {
             if (owner1 ? best == LLONG_MAX : out) {
                 h[v] = kHOut;
-                atomicOr(&cur[v >> 5], 1u << (v & 31));
+                bm_set(cur, g.n, v);
                 local++;
             } else if (best > hv) {
                 h[v] = best;
-                atomicOr(&cur[v >> 5], 1u << (v & 31));
+                bm_set(cur, g.n, v);
                 local++;
                 if (best > bound) atomicOr(&flags[1], 1);
             }
@@ -365,8 +395,7 @@ __global__ __launch_bounds__(256) void k_mark_top(DevGame g, const long long *__
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
         if (h[v] >= 0) { g.val[v] = EGS_TOP; local++; }
     }
-    local = wave_sum(local);
-    if ((threadIdx.x & 63) == 0 && local) atomicAdd(count, local);
+    block_add(local, count);
 }
 
 static void launch_bf_pass(int G, int grid, hipStream_t st, const DevGame &g, long long *h, long long bound,
@@ -433,7 +462,8 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
     const long long bound = scale * g.mg + (long long)g.n * (scale * wmax + 1);
     const int grid = grid_for(g.n, 256, 2048);
     if (hipMemsetAsync(cb.hist, 0, sizeof(int) * (2 * kBins + 8), st) != hipSuccess) return -1;
-    hipLaunchKernelGGL(k_gap_hist, dim3(grid), dim3(256), 0, st, g, wmax, cb.hist);
+    // few workgroups: each adds its non-empty bins to the global histogram with atomics
+    hipLaunchKernelGGL(k_gap_hist, dim3(std::min(grid, 256)), dim3(256), 0, st, g, wmax, cb.hist);
     if (hipMemcpyAsync(cb.pinned, cb.hist, sizeof(int) * 2 * kBins, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
     if (hipStreamSynchronize(st) != hipSuccess) return -1;
     int64_t cands[41];
@@ -442,8 +472,8 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
     int *flags = cb.hist + 2 * kBins;       // [0] closure failed, [1] bound exceeded, [2] marked, [3] members
     int *fl = cb.pinned + 2 * kBins;        // flags readback (4 ints), then the counter ring
     int *cnt = fl + 4;
-    const int64_t words = ((int64_t)g.n + 31) >> 5;
-    unsigned *bm[3] = {cb.bits, cb.bits + words, cb.bits + 2 * words};
+    const int64_t words = ((int64_t)g.n + 31) >> 5, bm_len = words + (words + 31) / 32;
+    unsigned *bm[3] = {cb.bits, cb.bits + bm_len, cb.bits + 2 * bm_len};
     // Passes are queued in chunks and read back once per chunk; a pass after the converged
     // one returns at once, so a chunk costs little beyond its real passes.  The first
     // chunk (closure and first pass, then more) is sized for the common case of a few
@@ -458,6 +488,7 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
         res.tried++;
         if (hipMemsetAsync(flags, 0, sizeof(int) * 4, st) != hipSuccess) return -1;
         if (hipMemsetAsync(cb.bfcnt, 0, sizeof(int) * kCntSlots, st) != hipSuccess) return -1;
+        if (hipMemsetAsync(bm[0] + words, 0, sizeof(unsigned) * (bm_len - words), st) != hipSuccess) return -1;
         hipLaunchKernelGGL(k_closure, dim3(grid), dim3(256), 0, st, g, B, last, bound, cb.h, bm[0], bm[1], cb.bfcnt,
                            flags);
         if (has_big)

@@ -226,7 +226,7 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
            o_tile = take(4 * (size_t)tiles), o_fill = take(4 * n1),
            o_stats = take(8 * 4 * (size_t)egs::kStatSlots), o_statsum = take(8 * 8),
            o_sigma = take(4 * n1), o_h = take(8 * n1), o_hist = take(4 * 256), o_bfcnt = take(4 * 64),
-           o_bits = take(4 * 3 * (n1 / 32 + 1));
+           o_bits = take(4 * 3 * (n1 / 32 + n1 / 1024 + 4));
     // phased dense rounds: slice offsets, slice-major edges, partial keys, scan tiles
     const bool phased = n_cap >= egs::kPhaseMinN;
     const size_t pn1 = (size_t)egs::kPhases * n_cap + 1;
@@ -536,6 +536,8 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
                                        s->stream));
                 HIP_TRY(hipStreamSynchronize(s->stream));
                 s->cert_gap = s->cert_every;
+                batch = 1;        // the fixpoint often follows at once: probe again early
+                nb = 1;
             } else {
                 s->cert_gap = std::min(s->cert_gap * 2, 1 << 16);
                 if (cr.cap_hit) s->cert_cap = std::min(s->cert_cap * 2, s->cert_max_passes);

@@ -89,7 +89,7 @@ struct CertBuffers {
     int *hist;        // device: 2 x 64 histogram bins + 8 flags
     long long *h;     // device: n longest-path potentials
     int *bfcnt;       // device: ring of 64 pass counters
-    unsigned *bits;   // device: 3 bitmaps of ceil(n / 32) words (vertices raised per pass)
+    unsigned *bits;   // device: 3 two-level bitmaps (ceil(n/32) words + a bit-per-word summary)
     int *pinned;      // host pinned: 2 x 64 + 4 + 64 ints
 };
 struct CertResult {

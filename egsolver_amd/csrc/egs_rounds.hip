@@ -270,10 +270,16 @@ __global__ __launch_bounds__(256) void k_init(DevGame g) {
             const int s = g.ptr[v], e = g.ptr[v + 1];
             bool all_neg = true, any_neg = false;
             if (e - s <= g.big_deg) {             // above big_deg: k_init_big
-                for (int k = s; k < e; k++) {
-                    const bool neg = g.edges[k].y < 0;
-                    all_neg &= neg;
-                    any_neg |= neg;
+                for (int k0 = s; k0 < e; k0 += 4) {
+                    int w[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) w[j] = k0 + j < e ? g.edges[k0 + j].y : 0;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        if (k0 + j >= e) break;
+                        all_neg &= w[j] < 0;
+                        any_neg |= w[j] < 0;
+                    }
                 }
                 act = (v < g.n0) ? all_neg : any_neg;
             }
