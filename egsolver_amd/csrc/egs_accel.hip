@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
     const int gl = threadIdx.x % G;
     const int gbase = (threadIdx.x & 63) & ~(G - 1);
     const long long scale = (long long)g.n + 1;
-    int local = 0;
+    int local = 0, removed = 0;
     for (int64_t base = (int64_t)blockIdx.x * kPer; base < g.n; base += (int64_t)gridDim.x * kPer) {
         const int v = (int)base + (int)threadIdx.x / G;
         const long long hv = v < g.n ? h[v] : kHOut;
@@ -314,6 +314,7 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
                 h[v] = kHOut;
                 bm_set(cur, g.n, v);
                 local++;
+                removed++;
             } else if (best > hv) {
                 h[v] = best;
                 bm_set(cur, g.n, v);
@@ -323,6 +324,7 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
         }
     }
     block_add(local, &cnt[p % kCntSlots]);
+    block_add(removed, &flags[0]);
 }
 
 // Pass p for the vertices above big_deg (launched after k_bf_pass for the same p), one
@@ -378,6 +380,7 @@ __global__ __launch_bounds__(256) void k_bf_big(DevGame g, long long *__restrict
                 h[v] = kHOut;
                 bm_set(cur, g.n, v);
                 local++;
+                atomicAdd(&flags[0], 1);
             } else if (best > hv) {
                 h[v] = best;
                 bm_set(cur, g.n, v);
@@ -449,10 +452,20 @@ int gap_candidates(const int *win, const int *flo, int wmax, int mg, int64_t *ou
 // every admissible gap threshold (smallest first, at most max_cands), then the vertices
 // that rose in round `last`.  Each is pruned to its greatest closed subset while at most
 // `cap` passes of the longest-path game run on it; the first that converges with a
-// non-empty set is set to TOP.  has_big: some vertex is above big_deg (its own kernels).  Returns 1 when vertices were
+// non-empty set is set to TOP (queued, not waited for).  has_big: some vertex is above
+// big_deg (its own kernels).  hist_ready: certify_prefetch was queued and has completed.  Returns 1 when vertices were
 // set to TOP (count in res.marked), 0 when no candidate gave a proof, < 0 on a HIP error.
+hipError_t certify_prefetch(const DevGame &g, const CertBuffers &cb, int wmax, hipStream_t st) {
+    hipError_t err = hipMemsetAsync(cb.hist, 0, sizeof(int) * (2 * kBins + 8), st);
+    if (err != hipSuccess) return err;
+    // few workgroups: each adds its non-empty bins to the global histogram with atomics
+    hipLaunchKernelGGL(k_gap_hist, dim3(std::min(grid_for(g.n, 256, 2048), 256)), dim3(256), 0, st, g, std::max(wmax, 1),
+                       cb.hist);
+    return hipMemcpyAsync(cb.pinned, cb.hist, sizeof(int) * 2 * kBins, hipMemcpyDeviceToHost, st);
+}
+
 int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int cap, int max_cands, bool has_big,
-                hipStream_t st, CertResult &res) {
+                bool hist_ready, hipStream_t st, CertResult &res) {
     constexpr int kBigGrid = 512;           // workgroups of the kernels for vertices above big_deg
     res = CertResult{};
     if (wmax <= 0 || !g.rise) return 0;
@@ -461,15 +474,14 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
     if ((double)scale * g.mg + (double)g.n * ((double)scale * wmax + 1) > 4.0e18) return 0;
     const long long bound = scale * g.mg + (long long)g.n * (scale * wmax + 1);
     const int grid = grid_for(g.n, 256, 2048);
-    if (hipMemsetAsync(cb.hist, 0, sizeof(int) * (2 * kBins + 8), st) != hipSuccess) return -1;
-    // few workgroups: each adds its non-empty bins to the global histogram with atomics
-    hipLaunchKernelGGL(k_gap_hist, dim3(std::min(grid, 256)), dim3(256), 0, st, g, wmax, cb.hist);
-    if (hipMemcpyAsync(cb.pinned, cb.hist, sizeof(int) * 2 * kBins, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
-    if (hipStreamSynchronize(st) != hipSuccess) return -1;
+    if (!hist_ready) {
+        if (certify_prefetch(g, cb, wmax, st) != hipSuccess) return -1;
+        if (hipStreamSynchronize(st) != hipSuccess) return -1;
+    }
     int64_t cands[41];
     int ncand = gap_candidates(cb.pinned, cb.pinned + kBins, wmax, g.mg, cands, std::min(max_cands, 40));
     cands[ncand++] = -1;                    // the vertices that rose in round `last`
-    int *flags = cb.hist + 2 * kBins;       // [0] closure failed, [1] bound exceeded, [2] marked, [3] members
+    int *flags = cb.hist + 2 * kBins;       // [0] removed, [1] bound exceeded, [2] marked, [3] members
     int *fl = cb.pinned + 2 * kBins;        // flags readback (4 ints), then the counter ring
     int *cnt = fl + 4;
     const int64_t words = ((int64_t)g.n + 31) >> 5, bm_len = words + (words + 31) / 32;
@@ -525,11 +537,10 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
             if (!diverged) res.cap_hit = 1;
             continue;
         }
+        res.marked = fl[3] - fl[0];         // members left after the removals
+        if (res.marked <= 0) continue;      // pruning emptied the candidate: try the next
         hipLaunchKernelGGL(k_mark_top, dim3(grid), dim3(256), 0, st, g, (const long long *)cb.h, flags + 2);
-        if (hipMemcpyAsync(fl + 2, flags + 2, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
-        if (hipStreamSynchronize(st) != hipSuccess) return -1;
-        res.marked = fl[2];
-        if (res.marked > 0) return 1;       // else pruning emptied the candidate: try the next
+        return hipGetLastError() == hipSuccess ? 1 : -1;
     }
     return 0;
 }

@@ -66,7 +66,11 @@ constexpr int kPinViol = 704;     // 1 x u64
 constexpr int kPinCert = 1024;    // certification readbacks (2 x 64 + 4 + 64 ints)
 constexpr int kPinForce = 1280;   // round number forced to a full sweep
 constexpr int kPinOne = 1281;     // the constant 1 (raises ring[12])
-constexpr int kProbeWant = 100;   // offset in a probe slot: ring[13] (a round wanted to go sparse)
+constexpr int kProbeWant = 100;   // offset in a probe slot: ring[13] (rounds that wanted to go sparse)
+// The transpose costs about ten dense rounds to build, and a sparse round saves at most one
+// dense round: build it once this many rounds have wanted it (a proof usually leaves only a
+// round or two, which do not pay for it).
+constexpr int kSparseWantRounds = 6;
 constexpr int kCertFirstCap = 64;     // longest-path passes per candidate at the first attempt
 constexpr int kCertCandidates = 8;    // gap thresholds tried per attempt (plus the last-round rise set)
 constexpr int kCertEvery = 12;        // default rounds before the first proof attempt
@@ -518,12 +522,15 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
             // pause: converged already?  else try to prove a climbing set is TOP
             HIP_TRY(hipMemcpyAsync(s->h_probe + kPinConv, s->d_lcnt + (s->next_round % 3) * egs::kListCounters,
                                    sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
+            // the attempt's value histogram rides on the same wait
+            egs::CertBuffers cb{s->d_hist, s->d_h, s->d_bfcnt, s->d_bits, s->h_probe + kPinCert};
+            const bool hist = s->wmax > 0;
+            if (hist) HIP_TRY(egs::certify_prefetch(s->g, cb, s->wmax, s->stream));
             HIP_TRY(hipStreamSynchronize(s->stream));
             if (list_total(s, s->h_probe + kPinConv) == 0) break;
-            egs::CertBuffers cb{s->d_hist, s->d_h, s->d_bfcnt, s->d_bits, s->h_probe + kPinCert};
             egs::CertResult cr;
             const int got = egs::certify_top(s->g, cb, s->wmax, (int)(s->next_round - 1), s->cert_cap,
-                                             kCertCandidates, s->sh.use_wave != 0, s->stream, cr);
+                                             kCertCandidates, s->sh.use_wave != 0, hist, s->stream, cr);
             if (got < 0) return fail(EGS_E_HIP, "certification: %s", hipGetErrorString(hipGetLastError()));
             s->st.cert_attempts++;
             s->st.cert_passes += cr.passes;
@@ -531,10 +538,10 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
             if (got > 0) {
                 s->st.certificates++;
                 s->st.top_marked += cr.marked;
-                s->h_probe[kPinForce] = (int)s->next_round;           // next round sweeps everything
+                // next round sweeps everything (the slot is rewritten only after later waits)
+                s->h_probe[kPinForce] = (int)s->next_round;
                 HIP_TRY(hipMemcpyAsync(s->d_ring + 11, s->h_probe + kPinForce, sizeof(int), hipMemcpyHostToDevice,
                                        s->stream));
-                HIP_TRY(hipStreamSynchronize(s->stream));
                 s->cert_gap = s->cert_every;
                 batch = 1;        // the fixpoint often follows at once: probe again early
                 nb = 1;
@@ -562,8 +569,9 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
         if (pending >= 0) {
             HIP_TRY(hipEventSynchronize(s->ev_probe[pending]));
             if (list_total(s, s->h_probe + kPinProbe + pending * 128) == 0) done = true;
-            // a round of that batch wanted to go sparse: build the transpose now
-            if (!done && !s->csc_built && s->h_probe[kPinProbe + pending * 128 + kProbeWant]) HIP_TRY(build_csc(s));
+            // enough rounds wanted to go sparse: build the transpose now
+            if (!done && !s->csc_built && s->h_probe[kPinProbe + pending * 128 + kProbeWant] >= kSparseWantRounds)
+                HIP_TRY(build_csc(s));
         }
         pending = slot;
         if (s->stop_flag && *s->stop_flag) break;

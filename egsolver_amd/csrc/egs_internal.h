@@ -36,7 +36,7 @@ struct DevGame {
     int *ring;            // [3..5] dbig counters, [6..8] dhuge counters, [9] nsbig, [10] nshuge,
                           // [11] a round that must sweep all vertices (after a TOP proof),
                           // [12] 1 once the transpose exists (sparse rounds allowed),
-                          // [13] 1 once a round would have gone sparse without it
+                          // [13] rounds that would have gone sparse without it
     const int *sbig;      // static: vertices with big_deg < deg <= huge_deg
     const int *shuge;     // static: vertices with deg > huge_deg
     int big_deg, huge_deg;
@@ -102,7 +102,9 @@ struct CertResult {
 };
 hipError_t launch_abs_wmax(const DevGame &g, int *out, hipStream_t st);
 int gap_candidates(const int *win, const int *flo, int wmax, int mg, int64_t *out, int max_out);
+// The value histogram of the next attempt, queued with its read-back into the pinned area.
+hipError_t certify_prefetch(const DevGame &g, const CertBuffers &cb, int wmax, hipStream_t st);
 int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int cap, int max_cands, bool has_big,
-                hipStream_t st, CertResult &res);
+                bool hist_ready, hipStream_t st, CertResult &res);
 
 }  // namespace egs

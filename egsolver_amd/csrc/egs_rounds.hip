@@ -147,7 +147,7 @@ __device__ __forceinline__ int decide_mode(const DevGame &g, int r, int total) {
 }
 
 // The sparse modes walk predecessor lists: until the transpose exists (ring[12], raised by
-// the host) such a round sweeps densely instead and says so in ring[13].
+// the host) such a round sweeps densely instead and counts itself in ring[13].
 __device__ __forceinline__ int gate_mode(const DevGame &g, int mode) {
     return (mode == MODE_QUEUE || mode == MODE_LIFTED) && g.ring[12] == 0 ? MODE_DENSE : mode;
 }
@@ -444,7 +444,7 @@ __global__ __launch_bounds__(256) void k_round(DevGame g, int r) {
     if (threadIdx.x == 0) {
         const int want = decide_mode(g, r, pre[kSub + 1]);
         smode = gate_mode(g, want);
-        if (blockIdx.x == 0 && smode != want) g.ring[13] = 1;
+        if (blockIdx.x == 0 && smode != want) g.ring[13]++;     // rounds that wanted the transpose
     }
     __syncthreads();
     const int mode = smode;
