@@ -6,7 +6,8 @@ operator runs as hand-written HIP kernels in ``lib/libegs.so`` behind a plain C-
 command-line driver sit on top.  This Python package is the host-side mirror used by
 tests and the benchmark.
 """
-from .games import Game, GenParams, config_game, generate, family_params, read_mpg, write_mpg, TOP  # noqa: F401
+from .games import (Game, GenParams, config_game, generate, family_params, read_mpg, write_mpg, parse_mpg,  # noqa: F401
+                    save_csr, load_csr, TOP)
 from .solver import Solver, solve, options, device_info  # noqa: F401
 from .batch import Batch, batch_options  # noqa: F401,E402
 from .serve import Server, Request, serve_options, run_served, run_plain, latency_stats  # noqa: F401,E402

@@ -105,6 +105,20 @@ class EgsGenReadOrder(ctypes.Structure):
     _fields_ = [("owner", _u8p), ("ptr", _i32p), ("succ", _i32p), ("w", _i32p)]
 
 
+class EgsCsr(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("n0", ctypes.c_int32),
+        ("m", ctypes.c_int64),
+        ("mg", ctypes.c_int32),
+        ("max_abs_w", ctypes.c_int32),
+        ("ptr", _i32p),
+        ("succ", _i32p),
+        ("w", _i32p),
+        ("ext_of_int", _i32p),
+    ]
+
+
 class EgsMpg(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int32),
@@ -120,6 +134,7 @@ class EgsMpg(ctypes.Structure):
         ("max_outdeg", ctypes.c_int32),
         ("low_outdeg", ctypes.c_int32 * 4),
         ("names", ctypes.POINTER(ctypes.c_char_p)),
+        ("chunks", ctypes.c_int32),
     ]
 
 
@@ -156,6 +171,9 @@ EGS_HOST_SIGNATURES = {
                                     ctypes.POINTER(EgsMpg), ctypes.c_char_p, ctypes.c_int]),
     "egs_mpg_read_mem": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.POINTER(EgsMpg), ctypes.c_char_p, ctypes.c_int]),
+    "egs_mpg_read_mem_par": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.POINTER(EgsMpg), ctypes.c_char_p, ctypes.c_int,
+                                            ctypes.c_int]),
     "egs_mpg_free": (None, [ctypes.POINTER(EgsMpg)]),
     "egs_mpg_internalise": (ctypes.c_int, [ctypes.POINTER(EgsMpg), _vp, _vp, _vp, _vp]),
     "egs_internalise": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -167,6 +185,10 @@ EGS_HOST_SIGNATURES = {
     "egs_gen_family_params": (None, [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(EgsGenParams)]),
     "egs_rng_test_stream": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _vp]),
     "egs_write_mpg": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int]),
+    "egs_csr_save": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                                    _vp, _vp, _vp, _vp, ctypes.c_char_p, ctypes.c_int]),
+    "egs_csr_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(EgsCsr), ctypes.c_char_p, ctypes.c_int]),
+    "egs_csr_free": (None, [ctypes.POINTER(EgsCsr)]),
 }
 
 _egs = None

@@ -210,10 +210,15 @@ int egs_internalise(int n, const uint8_t *owner, const int32_t *rptr, const int3
     int64_t e = 0;
     for (int i = 0; i < n; i++) {
         ptr[i] = (int32_t)e;
-        int r = ext_of_int[i];
-        for (int j = rptr[r]; j < rptr[r + 1]; j++) { succ[e] = rev[rsucc[j]]; w[e] = rw[j]; e++; }
+        e += rptr[ext_of_int[i] + 1] - rptr[ext_of_int[i]];
     }
     ptr[n] = (int32_t)e;
+#pragma omp parallel for schedule(dynamic, 4096) if (e > (1 << 16))
+    for (int i = 0; i < n; i++) {
+        const int r = ext_of_int[i];
+        int64_t at = ptr[i];
+        for (int j = rptr[r]; j < rptr[r + 1]; j++, at++) { succ[at] = rev[rsucc[j]]; w[at] = rw[j]; }
+    }
     free(rev);
     return n0;
 }
@@ -221,13 +226,15 @@ int egs_internalise(int n, const uint8_t *owner, const int32_t *rptr, const int3
 /* MG = sum over vertices of max(0, max_e -w_e) (src/egsolver.c:283-304).  Returns -1
  * when the sum would pass INT32_MAX (the reference aborts on that input). */
 int egs_compute_mg(int n, const int32_t *ptr, const int32_t *w, int32_t *mg_out) {
+    /* every term is >= 0, so the running sum passes INT32_MAX iff the total does */
     int64_t mg = 0;
+#pragma omp parallel for schedule(static) reduction(+ : mg) if (n > (1 << 16))
     for (int v = 0; v < n; v++) {
-        int32_t worst = 0;
-        for (int e = ptr[v]; e < ptr[v + 1]; e++) if (-w[e] > worst) worst = -w[e];
+        int64_t worst = 0;
+        for (int e = ptr[v]; e < ptr[v + 1]; e++) if (-(int64_t)w[e] > worst) worst = -(int64_t)w[e];
         mg += worst;
-        if (mg > INT32_MAX) return -1;
     }
+    if (mg > INT32_MAX) return -1;
     *mg_out = (int32_t)mg;
     return 0;
 }

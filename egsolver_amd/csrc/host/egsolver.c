@@ -106,6 +106,8 @@ static void usage(void) {
     fprintf(stderr, "  --shuffling N\n\tLanes per low-degree vertex, N=1,2,4,8,16,32 (default: chosen from the degrees).\n");
     fprintf(stderr, "  --timeout [S]\n\tStop after about S seconds with partial values (default S=%d).\n",
             DEFAULT_TIMEOUT_SEC);
+    fprintf(stderr, "  --save-csr FILE\n\tAfter reading, save the game in its internal numbering as a binary cache.\n");
+    fprintf(stderr, "  --load-csr FILE\n\tRead the game from a binary cache made by --save-csr instead of the input.\n");
     fprintf(stderr, "\n Output:\n");
     fprintf(stderr, "  --onelineout\n\tSolution in a single text line.\n");
     fprintf(stderr, "  --noout\n\tDo not print the solution.\n");
@@ -145,6 +147,8 @@ static void bad(const char *opt, const char *val) {
 
 static const char *input_path = NULL;
 static int weight_mode = EGS_W_KEEP;
+static const char *save_csr_path = NULL, *load_csr_path = NULL;   /* binary game cache */
+static double postt = 0;   /* post-parsing seconds (0 when the game comes from the cache) */
 
 static void setconfig(int argc, char **argv) {
     memset(&configuration, 0, sizeof(configuration));
@@ -196,6 +200,10 @@ static void setconfig(int argc, char **argv) {
                 exit(1);
             }
             fclose(f);
+        } else if (!strcmp(a, "--save-csr") || !strcmp(a, "--load-csr")) {
+            if (!has_param(i + 1, argc, argv)) bad(a, "(missing)");
+            if (a[2] == 's') save_csr_path = argv[++i];
+            else load_csr_path = argv[++i];
         } else if (!strcmp(a, "--shuffling")) {
             if (has_param(i + 1, argc, argv)) {
                 int k = to_int(argv[++i], "--shuffling");
@@ -310,35 +318,8 @@ static void on_alarm(int signo) {
     }
 }
 
-int main(int argc, char **argv) {
-    prog = argv[0];
-    setconfig(argc, argv);
-    memset(&statistics, 0, sizeof(statistics));
-
-    char host[256] = "";
-    gethostname(host, sizeof(host));
-    printf("Process stats:\n\tProcess: PID=%d  running on %s\n", getpid(), host);
-    printf("\tCommand line: ");
-    for (int i = 0; i < argc; i++) printf("%s ", argv[i]);
-    printf("\n");
-    fflush(stdout);
-
-    int device_count = 0;
-    checkDevice(&device_count, &configuration);
-    configuration.deviceCount = (unsigned)device_count;
-    {
-        egs_device_info info;
-        egs_device_query(configuration.deviceid, &info);
-        printf("GPU stats:\n\t%d HIP devices detected.\n", device_count);
-        printf("\tSelect device number %d  name: %s\n", configuration.deviceid, configuration.devicename);
-        printf("\tWavefront size: %d\n", configuration.warpSize);
-        printf("\tGPU clock: %.0f MHz\n", (float)configuration.clockRate * 1e-3f);
-        printf("\tDevice memory: %.0f MB\n", (double)configuration.deviceProp_totalGlobalMem / 1048576.0);
-        printf("\tECC enabled: %s\n", configuration.ECCEnabled ? "yes" : "no");
-        printf("\tHIP Driver Version / Runtime Version   %d / %d\n\n", info.driver_version, info.runtime_version);
-    }
-
-    /* parse + remap */
+/* Read the .mpg input and build the internal CSR (src/egsolver.c:107-315). */
+static void load_from_mpg(void) {
     double t0 = now_s();
     printf("Parsing input (reading %s)...\n", configuration.stdinputsource ? "stdin" : configuration.filename);
     fflush(stdout);
@@ -428,8 +409,111 @@ int main(int argc, char **argv) {
     egs_mpg_free(&g);
     free(outdeg);
     free(indeg);
-    double postt = now_s() - t1;
+    postt = now_s() - t1;
     printf("Post-parsing completed (postparsing time %lf).\n", postt);
+}
+
+/* The game from a binary cache made by --save-csr: internal numbering, MG and external ids
+ * as saved, so the solve and the output are those of the run that saved it. */
+static void load_from_cache(void) {
+    double t0 = now_s();
+    printf("Reading the game cache %s...\n", load_csr_path);
+    fflush(stdout);
+    egs_csr c;
+    char err[512];
+    if (egs_csr_load(load_csr_path, &c, err, sizeof(err)) != 0) {
+        fprintf(stderr, "%s\n", err);
+        exit(EXIT_FAILURE);
+    }
+    const int n = c.n;
+    const long m = (long)c.m;
+    num_nodi = counter_nodi = n;
+    num_archi = (int)m;
+    max_nodo = n - 1;
+    max_pesi = c.max_abs_w;
+    counter_nodi0 = c.n0;
+    min_outdegree = INT_MAX;
+    max_outdegree = 0;
+    memset(numLowOutDegree, 0, sizeof(numLowOutDegree));
+    for (int i = 0; i < n; i++) {
+        const int d = c.ptr[i + 1] - c.ptr[i];
+        if (d < min_outdegree) min_outdegree = d;
+        if (d > max_outdegree) max_outdegree = d;
+        if (d < 4) numLowOutDegree[d]++;
+    }
+    mapping = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    revmapping = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    nomeExt_of_nomeInt = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    for (int i = 0; i < n; i++) {
+        mapping[i] = revmapping[i] = i;
+        nomeExt_of_nomeInt[i] = c.ext_of_int[i];
+    }
+    alloca_memoria_host();
+    alloca_memoria_device();
+    memcpy(host_csrPtrInSuccLists, c.ptr, sizeof(int) * (size_t)(n + 1));
+    memcpy(host_csrSuccLists, c.succ, sizeof(int) * (size_t)m);
+    memcpy(host_csrPesiArchi, c.w, sizeof(int) * (size_t)m);
+    MG_pesi = c.mg;
+    avg_outdegree = (double)m / (double)n;
+    double sq = 0;
+    for (int i = 0; i < n; i++) {
+        const double d = (double)(c.ptr[i + 1] - c.ptr[i]) - avg_outdegree;
+        sq += d * d;
+    }
+    stddev_outdegree = sqrt(sq / n);
+    egs_csr_free(&c);
+    statistics.inputtime = now_s() - t0;
+    printf("Cache read (reading time %lf).\n", statistics.inputtime);
+}
+
+int main(int argc, char **argv) {
+    prog = argv[0];
+    setconfig(argc, argv);
+    memset(&statistics, 0, sizeof(statistics));
+
+    char host[256] = "";
+    gethostname(host, sizeof(host));
+    printf("Process stats:\n\tProcess: PID=%d  running on %s\n", getpid(), host);
+    printf("\tCommand line: ");
+    for (int i = 0; i < argc; i++) printf("%s ", argv[i]);
+    printf("\n");
+    fflush(stdout);
+
+    int device_count = 0;
+    checkDevice(&device_count, &configuration);
+    configuration.deviceCount = (unsigned)device_count;
+    {
+        egs_device_info info;
+        egs_device_query(configuration.deviceid, &info);
+        printf("GPU stats:\n\t%d HIP devices detected.\n", device_count);
+        printf("\tSelect device number %d  name: %s\n", configuration.deviceid, configuration.devicename);
+        printf("\tWavefront size: %d\n", configuration.warpSize);
+        printf("\tGPU clock: %.0f MHz\n", (float)configuration.clockRate * 1e-3f);
+        printf("\tDevice memory: %.0f MB\n", (double)configuration.deviceProp_totalGlobalMem / 1048576.0);
+        printf("\tECC enabled: %s\n", configuration.ECCEnabled ? "yes" : "no");
+        printf("\tHIP Driver Version / Runtime Version   %d / %d\n\n", info.driver_version, info.runtime_version);
+    }
+
+    if (load_csr_path) {
+        load_from_cache();
+    } else {
+        load_from_mpg();
+    }
+    if (save_csr_path) {
+        int *ext = (int *)malloc(sizeof(int) * (size_t)num_nodi);
+        for (int i = 0; i < num_nodi; i++) ext[i] = nomeExt_of_nomeInt[mapping[i]];
+        char cerrbuf[512];
+        if (egs_csr_save(save_csr_path, num_nodi, counter_nodi0, num_archi, MG_pesi, host_csrPtrInSuccLists,
+                         host_csrSuccLists, host_csrPesiArchi, ext, cerrbuf, sizeof(cerrbuf)) != 0) {
+            fprintf(stderr, "%s\n", cerrbuf);
+            exit(EXIT_FAILURE);
+        }
+        free(ext);
+        printf("Game saved to %s\n", save_csr_path);
+    }
+
+    printf("\nInput stats:\n\tNodes: %d (%d+%d)\n\tEdges: %d\n\tMax node Id: %d\n", num_nodi, counter_nodi0,
+           counter_nodi - counter_nodi0, num_archi, max_nodo);
 
     printf("\nInput stats:\n\tNodes: %d (%d+%d)\n\tEdges: %d\n\tMax node Id: %d\n", num_nodi, counter_nodi0,
            counter_nodi - counter_nodi0, num_archi, max_nodo);

@@ -134,13 +134,45 @@ def write_mpg(path, owner, rptr, rsucc, rw, gz=False):
         raise OSError(f"cannot write {path}")
 
 
-def read_mpg(path=None, text=None, weight_mode=0, rnd_lo=-1, rnd_hi=1) -> Game:
-    """Read a .mpg arena with the package's C reader and internalise it."""
+def _read_into(g, err, path, text, weight_mode, rnd_lo, rnd_hi, threads):
+    if text is not None:
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        if threads == 1:
+            return N.host().egs_mpg_read_mem(data, len(data), weight_mode, rnd_lo, rnd_hi, ctypes.byref(g), err, 512)
+        return N.host().egs_mpg_read_mem_par(data, len(data), weight_mode, rnd_lo, rnd_hi, ctypes.byref(g), err, 512,
+                                             threads)
+    return N.host().egs_mpg_read(path.encode() if path else None, weight_mode, rnd_lo, rnd_hi, ctypes.byref(g), err, 512)
+
+
+def parse_mpg(text, weight_mode=0, rnd_lo=-1, rnd_hi=1, threads=1) -> dict:
+    """The reader's result in read order, before the owner partition: threads=1 runs the
+    sequential reader, anything else the parallel one (0 = OpenMP's default)."""
+    g = N.EgsMpg()
+    err = ctypes.create_string_buffer(512)
+    rc = _read_into(g, err, None, text, weight_mode, rnd_lo, rnd_hi, threads)
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    try:
+        n, m = g.n, g.m
+        arr = lambda p, k: np.ctypeslib.as_array(p, shape=(k,)).copy() if k else np.zeros(0, np.int32)
+        out = {"n": n, "m": m, "ext_of_read": arr(g.ext_of_read, n), "owner": arr(g.owner, n),
+               "rptr": arr(g.rptr, n + 1), "rsucc": arr(g.rsucc, m), "rw": arr(g.rw, m),
+               "max_abs_w": g.max_abs_w, "min_outdeg": g.min_outdeg, "max_outdeg": g.max_outdeg,
+               "low_outdeg": list(g.low_outdeg),
+               "names": [g.names[i].decode() if g.names[i] else None for i in range(n)],
+               "chunks": g.chunks}
+    finally:
+        N.host().egs_mpg_free(ctypes.byref(g))
+    return out
+
+
+def read_mpg(path=None, text=None, weight_mode=0, rnd_lo=-1, rnd_hi=1, threads=1) -> Game:
+    """Read a .mpg arena with the package's C reader and internalise it.  A file is read by
+    the parallel reader; `text` by the sequential one unless `threads` != 1."""
     g = N.EgsMpg()
     err = ctypes.create_string_buffer(512)
     if text is not None:
-        data = text.encode() if isinstance(text, str) else bytes(text)
-        rc = N.host().egs_mpg_read_mem(data, len(data), weight_mode, rnd_lo, rnd_hi, ctypes.byref(g), err, 512)
+        rc = _read_into(g, err, None, text, weight_mode, rnd_lo, rnd_hi, threads)
     else:
         rc = N.host().egs_mpg_read(path.encode() if path else None, weight_mode, rnd_lo, rnd_hi,
                                    ctypes.byref(g), err, 512)
@@ -158,3 +190,28 @@ def read_mpg(path=None, text=None, weight_mode=0, rnd_lo=-1, rnd_hi=1) -> Game:
     finally:
         N.host().egs_mpg_free(ctypes.byref(g))
     return Game(n=n, n0=int(n0), ptr=ptr, succ=succ, w=w, mg=compute_mg(n, ptr, w), ext_of_int=ext)
+
+
+def save_csr(g: Game, path: str):
+    """Binary cache of a game in its internal numbering (egs_csr_save)."""
+    err = ctypes.create_string_buffer(512)
+    arrs = [np.ascontiguousarray(a, np.int32) for a in (g.ptr, g.succ, g.w, g.ext_of_int)]
+    rc = N.host().egs_csr_save(path.encode(), g.n, g.n0, g.m, g.mg, *[_p(a) for a in arrs], err, 512)
+    if rc != 0:
+        raise OSError(err.value.decode())
+
+
+def load_csr(path: str) -> Game:
+    """A game saved by save_csr; raises ValueError on a damaged or foreign file."""
+    c = N.EgsCsr()
+    err = ctypes.create_string_buffer(512)
+    if N.host().egs_csr_load(path.encode(), ctypes.byref(c), err, 512) != 0:
+        raise ValueError(err.value.decode())
+    try:
+        n, m = c.n, c.m
+        arr = lambda p, k: np.ctypeslib.as_array(p, shape=(k,)).copy() if k else np.zeros(0, np.int32)
+        g = Game(n=n, n0=c.n0, ptr=arr(c.ptr, n + 1), succ=arr(c.succ, m), w=arr(c.w, m), mg=c.mg,
+                 ext_of_int=arr(c.ext_of_int, n))
+    finally:
+        N.host().egs_csr_free(ctypes.byref(c))
+    return g

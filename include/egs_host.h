@@ -35,6 +35,7 @@ typedef struct egs_mpg {
     int32_t min_outdeg, max_outdeg;
     int32_t low_outdeg[4];
     char **names;         /* optional "string" per vertex read, or NULL entries */
+    int32_t chunks;       /* chunks the text was parsed in (1: the sequential reader) */
 } egs_mpg;
 
 /* Parse a .mpg file (plain or gzip, path NULL = stdin).  Weight mode and the
@@ -45,6 +46,12 @@ int egs_mpg_read(const char *path, int weight_mode, int rnd_lo, int rnd_hi, egs_
                  char *errbuf, int errlen);
 int egs_mpg_read_mem(const char *text, int64_t len, int weight_mode, int rnd_lo, int rnd_hi,
                      egs_mpg *out, char *errbuf, int errlen);
+/* The same result as egs_mpg_read_mem (values, order, names, messages), parsed by up to
+ * `threads` threads (0 = OpenMP's default) over line-aligned chunks of the text.  Inputs
+ * under 1 MB, random weights and anything the chunks cannot parse on their own (a cut
+ * inside a statement, an error) go through egs_mpg_read_mem.  egs_mpg_read uses it. */
+int egs_mpg_read_mem_par(const char *text, int64_t len, int weight_mode, int rnd_lo, int rnd_hi,
+                         egs_mpg *out, char *errbuf, int errlen, int threads);
 void egs_mpg_free(egs_mpg *g);
 
 /* Internal numbering of a parsed game: owner-0 vertices first, both blocks in read order
@@ -102,6 +109,26 @@ void egs_gen_family_params(int family, uint64_t seed, int index, egs_gen_params 
 
 /* Raw PRNG output, for the generator's own known-answer test. */
 uint64_t egs_rng_test_stream(uint64_t seed, uint64_t stream, int count, uint64_t *out);
+
+/* ---- binary game cache (no reference counterpart) ------------------------------------ */
+
+/* A game in its internal numbering, as egs_csr_load returns it (arrays owned by it). */
+typedef struct egs_csr {
+    int32_t n, n0;
+    int64_t m;
+    int32_t mg, max_abs_w;
+    int32_t *ptr;         /* n+1 */
+    int32_t *succ;        /* m */
+    int32_t *w;           /* m */
+    int32_t *ext_of_int;  /* n: external id of each internal vertex */
+} egs_csr;
+
+/* Save / load a game in the internal numbering with a checksum; a load refuses a file
+ * whose header, size, checksum or graph does not hold up.  0 or -1 with a message. */
+int egs_csr_save(const char *path, int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *ptr,
+                 const int32_t *succ, const int32_t *w, const int32_t *ext_of_int, char *errbuf, int errlen);
+int egs_csr_load(const char *path, egs_csr *out, char *errbuf, int errlen);
+void egs_csr_free(egs_csr *g);
 
 /* Write a game in .mpg format (read order, external ids 0..n-1); gz != 0 for gzip. */
 int egs_write_mpg(const char *path, int n, const uint8_t *owner, const int32_t *rptr,
