@@ -268,6 +268,63 @@ def serving(E, args):
             "capacity_speedup_vs_plain": cap / plain_cap, "arena_gb": args.serve_arena_gb}
 
 
+def ingest(E, args):
+    """Host ingest of a game given as .mpg text: the sequential reader, the parallel one,
+    the owner partition, and the binary game cache (save and load).  The game is config 2
+    at a quarter of its vertices, written to a temporary file first."""
+    import ctypes
+    import tempfile
+    from egsolver_amd import games as G
+    from egsolver_amd import _native as N
+    p = E.GenParams(n=max(16, int((1 << 22) * args.ingest_scale)), seed=2, deg_kind=0, deg_lo=8, deg_hi=8, wmax=100)
+    owner, rptr, rsucc, rw = G.generate_read_order(p)
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        path = os.path.join(d, "game.mpg")
+        E.write_mpg(path, owner, rptr, rsucc, rw)
+        with open(path, "rb") as f:
+            text = f.read()
+        m = int(rptr[-1])
+
+        def best_of(fn, k=3):
+            ts = []
+            for _ in range(k):
+                t = time.perf_counter()
+                r = fn()
+                ts.append(time.perf_counter() - t)
+            return min(ts), r
+
+        def parse(threads):
+            gm = N.EgsMpg()
+            err = ctypes.create_string_buffer(512)
+            fn = N.host().egs_mpg_read_mem if threads == 1 else N.host().egs_mpg_read_mem_par
+            extra = () if threads == 1 else (threads,)
+            if fn(text, len(text), 0, -1, 1, ctypes.byref(gm), err, 512, *extra) != 0:
+                raise SystemExit(err.value.decode())
+            chunks = gm.chunks
+            N.host().egs_mpg_free(ctypes.byref(gm))
+            return chunks
+
+        t_seq, _ = best_of(lambda: parse(1))
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        t_par, chunks = best_of(lambda: parse(threads))
+        t_file, g = best_of(lambda: E.read_mpg(path))          # parallel read + partition + MG
+        cache = os.path.join(d, "game.egs")
+        t_save, _ = best_of(lambda: E.save_csr(g, cache))
+        t_load, h = best_of(lambda: E.load_csr(cache))
+        assert np.array_equal(h.succ, g.succ) and np.array_equal(h.ptr, g.ptr)
+        cache_bytes = os.path.getsize(cache)
+    out = {"workload": f"config-2 distribution at n={p.n} (m={m}) as .mpg text, {len(text) / 1e6:.0f} MB",
+           "text_mb": len(text) / 1e6, "edges": m,
+           "sequential_read_s": t_seq, "parallel_read_s": t_par, "parallel_chunks": chunks,
+           "read_mb_per_s": {"sequential": len(text) / t_seq / 1e6, "parallel": len(text) / t_par / 1e6},
+           "file_to_game_s": t_file, "cache_save_s": t_save, "cache_load_s": t_load,
+           "cache_gb_per_s": cache_bytes / t_load / 1e9,
+           "edges_per_s": {"sequential_read": m / t_seq, "parallel_read": m / t_par, "cache_load": m / t_load}}
+    log(f"ingest: {len(text) / 1e6:.0f} MB text, read {t_seq:.2f}s sequential / {t_par:.2f}s in {chunks} chunks, "
+        f"file->game {t_file:.2f}s, cache load {t_load:.3f}s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,6 +349,8 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.75)
     ap.add_argument("--serve-arena-gb", type=float, default=24.0)
     ap.add_argument("--slice-kedges", type=int, default=0)
+    ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--ingest-scale", type=float, default=0.25, help="share of config 2's vertices for the ingest leg")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -341,6 +400,8 @@ def main():
         line["batched"], _ = batched(E, args)
     if not args.no_serve:
         line["serving"] = serving(E, args)
+    if not args.no_ingest:
+        line["ingest"] = ingest(E, args)
     print(json.dumps(line), flush=True)
 
 
