@@ -491,8 +491,10 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
     // chunk (closure and first pass, then more) is sized for the common case of a few
     // passes; later chunks are longer.
     const int first_chunk = 6, chunk = 8;
+    // lanes per vertex: about two edges each (on the 2^22-vertex game, 4 lanes took the
+    // passes 0.1 ms below 8 lanes)
     int G = 1;
-    while (G < 16 && G < (double)g.m / g.n) G <<= 1;
+    while (G < 16 && 2 * G < (double)g.m / g.n) G <<= 1;
     const int bf_grid = grid_for((int64_t)g.n * G, 256, 4096);
     for (int c = 0; c < ncand; c++) {
         const int B = (int)cands[c];

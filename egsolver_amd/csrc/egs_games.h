@@ -51,7 +51,7 @@ struct GameDesc {
 
 constexpr int kGameFirstCap = 32;      // longest-path passes per candidate at a game's first attempt
 constexpr int kGameCandidates = 8;     // gap thresholds tried per attempt (plus the last-rise set)
-constexpr int kGameCertifyEvery = 8;   // default sweeps before the first proof attempt
+constexpr int kGameCertifyEvery = 4;   // default sweeps before the first proof attempt
 
 struct CertArgs {
     int every;             // sweeps between proof attempts; <= 0 = no proofs

@@ -173,7 +173,7 @@ typedef struct egs_batch_options {
                                    degree * this / 16; 0 = 2 */
     int32_t group_min, group_max; /* clamp of the lanes per vertex; 0 = 1 / 64 */
     int64_t max_sweeps;         /* per-game sweep cap; <= 0 = none */
-    int32_t certify_every;      /* sweeps between TOP proofs per game; 0 = 8, < 0 = never */
+    int32_t certify_every;      /* sweeps between TOP proofs per game; 0 = 4, < 0 = never */
     int32_t cert_max_passes;    /* ceiling of the longest-path passes per proof candidate; 0 = 1024 */
 } egs_batch_options;
 
@@ -222,7 +222,7 @@ typedef struct egs_serve_options {
     int32_t lanes_per_edge_x16; /* 0 = 2 */
     int32_t group_min, group_max;
     int32_t slice_sweeps;       /* cap of complete sweeps per game per slice; 0 = none */
-    int32_t certify_every;      /* 0 = 8, < 0 = never */
+    int32_t certify_every;      /* 0 = 4, < 0 = never */
     int32_t cert_max_passes;    /* 0 = 1024 */
     int32_t slice_kedges;       /* edge budget of each workgroup per slice, in thousands of
                                    edges (bounds the slice time; sweeps resume part-way);
