@@ -154,6 +154,7 @@ EGS_SIGNATURES = {
     "egs_solver_iterate": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.POINTER(EgsStats)]),
     "egs_solver_stats": (ctypes.c_int, [_vp, ctypes.POINTER(EgsStats)]),
     "egs_solver_read_values": (ctypes.c_int, [_vp, _vp]),
+    "egs_solver_read_winning": (ctypes.c_int, [_vp, _vp]),
     "egs_solver_device_values": (_vp, [_vp]),
     "egs_solver_read_csc": (ctypes.c_int, [_vp, _vp, _vp]),
     "egs_solver_stream": (_vp, [_vp]),

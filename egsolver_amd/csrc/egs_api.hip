@@ -606,6 +606,17 @@ int egs_solver_read_values(egs_solver *s, int32_t *host_values) {
     return EGS_OK;
 }
 
+int egs_solver_read_winning(egs_solver *s, uint32_t *host_bits) {
+    if (!s || !s->loaded || !host_bits) return fail(EGS_E_STATE, "egs_solver_read_winning: nothing to read");
+    HIP_TRY(hipSetDevice(s->device));
+    // the proof bitmaps are idle outside an attempt: the first holds ceil(n/32) words
+    HIP_TRY(egs::launch_winning(s->d_val, s->g.n, s->d_bits, s->stream));
+    HIP_TRY(hipMemcpyAsync(host_bits, s->d_bits, sizeof(uint32_t) * (((size_t)s->g.n + 31) / 32), hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return EGS_OK;
+}
+
 const int32_t *egs_solver_device_values(egs_solver *s) { return s ? s->d_val : nullptr; }
 
 void *egs_solver_stream(egs_solver *s) { return s ? (void *)s->stream : nullptr; }

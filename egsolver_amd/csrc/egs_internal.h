@@ -82,6 +82,7 @@ hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int r, hipStrea
 hipError_t launch_sum_stats(const DevGame &g, int nslots, unsigned long long *out, hipStream_t st);
 hipError_t launch_degree_stats(const int *ptr, int n, int *out, hipStream_t st);
 hipError_t launch_violations(const DevGame &g, unsigned long long *out, hipStream_t st);
+hipError_t launch_winning(const int *val, int n, unsigned *out, hipStream_t st);
 int grid_for(int64_t work, int threads, int cap);
 
 // Certified TOP detection (egs_accel.hip).

@@ -215,6 +215,21 @@ __global__ __launch_bounds__(256) void k_violations(DevGame g, unsigned long lon
     if ((threadIdx.x & 63) == 0 && bad) atomicAdd(out, bad);
 }
 
+// Player 0's winning set as a bitmap: bit v of word v/32 is set when f(v) < TOP.
+__global__ __launch_bounds__(256) void k_winning(const int *__restrict__ val, int n, unsigned *__restrict__ out) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v - (threadIdx.x & 63) < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long b = __ballot(v < n && val[v] != EGS_TOP);
+        const int lane = threadIdx.x & 63;
+        if ((lane == 0 || lane == 32) && v < n) out[v >> 5] = (unsigned)(b >> lane);
+    }
+}
+
+hipError_t launch_winning(const int *val, int n, unsigned *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_winning, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, val, n, out);
+    return hipGetLastError();
+}
+
 __global__ void k_degree_max(const int *__restrict__ ptr, int n, int *__restrict__ out) {
     int mx = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;

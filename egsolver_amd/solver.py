@@ -79,6 +79,12 @@ class Solver:
         N.check(N.egs().egs_solver_read_values(self.h, out.ctypes.data), "egs_solver_read_values")
         return out
 
+    def winning(self):
+        """Player 0's winning set, {v : value < TOP}, as a bool array (internal numbering)."""
+        words = np.empty((self.n + 31) // 32, np.uint32)
+        N.check(N.egs().egs_solver_read_winning(self.h, words.ctypes.data), "egs_solver_read_winning")
+        return np.unpackbits(words.view(np.uint8), bitorder="little")[: self.n].astype(bool)
+
     def csc(self):
         cptr = np.empty(self.n + 1, np.int32)
         cpred = np.empty(self.m, np.int32)

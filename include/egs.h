@@ -141,6 +141,10 @@ int egs_solver_stats(egs_solver *s, egs_stats *st);
 int egs_solver_load_packed(egs_solver *s, int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *d_ptr,
                            const uint32_t *d_edges, int32_t shift, int32_t wmax, int32_t max_outdeg);
 int egs_solver_read_values(egs_solver *s, int32_t *host_values);
+/* No reference counterpart.  Player 0's winning set {v : value < TOP} as ceil(n/32) words,
+ * bit v%32 of word v/32, internal numbering (the reference derives it from the printed
+ * values). */
+int egs_solver_read_winning(egs_solver *s, uint32_t *host_bits);
 const int32_t *egs_solver_device_values(egs_solver *s);
 /* Device arrays of the transposed graph (for inspection): cptr[n+1], cpred[m]. */
 int egs_solver_read_csc(egs_solver *s, int32_t *cptr, int32_t *cpred);

@@ -212,6 +212,22 @@ def test_violations_count_with_hubs(oracle):
             assert s.violations() == want, k
 
 
+def test_winning_set_bitmap(oracle):
+    """The device winning-set bitmap is {v : value < TOP} of the exact values, for sizes that
+    are and are not multiples of the 32-bit word."""
+    import egsolver_amd as E
+    for n, seed in ((2000, 1), (2017, 2), (64, 3), (33, 4), (5003, 5)):
+        g = E.generate(E.GenParams(n=n, seed=seed, deg_kind=1, deg_lo=1, deg_hi=6, wmax=20))
+        _say(f"winning set n {g.n}")
+        ref = lfp(oracle, to_oracle_game(g))
+        with E.Solver(g.n, g.m) as s:
+            s.load(g)
+            s.prepare()
+            s.iterate()
+            assert np.array_equal(s.values(), ref)
+            assert np.array_equal(s.winning(), ref != TOP)
+
+
 def test_phased_equals_single_sweep_full_size():
     """Config 2 at full size: phased and single-sweep dense rounds end on the same values
     (no oracle finishes this size; the device fixpoint check covers it)."""
