@@ -108,6 +108,9 @@ __device__ __forceinline__ bool bm_test(const unsigned *bm, int n, int u) {
 // and zeroes slot (p + 1) % kCntSlots for the pass after it.
 constexpr int kCntSlots = 64;
 
+constexpr int kBigU = 4;   // edges in flight per lane in the workgroup-per-vertex kernels
+constexpr int kClosureU = 4;   // edges in flight per thread in k_closure (one thread per vertex)
+
 // Closure check, member count and the first pass of the longest-path game, in one sweep
 // over the values.  A member that breaks closure (an owner-0 edge leaving D and not to TOP,
 // or an owner-1 vertex with no edge into D or TOP) is removed from D; the removal is a
@@ -144,18 +147,27 @@ __global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, lon
             const bool owner1 = v >= g.n0;
             long long best = owner1 ? LLONG_MAX : LLONG_MIN;
             bool any = false, all = true;
-            for (int k = g.ptr[v]; k < g.ptr[v + 1]; k++) {
-                const int2 ew = g.edges[k];
-                const int xu = g.val[ew.x];
-                if (xu == EGS_TOP) {                 // ends the path: -infinity for the game
-                    any = true;
-                    if (owner1) best = LLONG_MIN;
-                } else if (in_d_x(g, ew.x, xu, B, last)) {
-                    any = true;
-                    const long long c = scale * ew.y + 1 + scale * ((long long)g.mg - xu);
-                    best = owner1 ? min(best, c) : max(best, c);
-                } else {
-                    all = false;
+            const int ke = g.ptr[v + 1];
+            for (int k0 = g.ptr[v]; k0 < ke; k0 += kClosureU) {   // kClosureU gathers in flight
+                int2 ew[kClosureU];
+                int xu[kClosureU];
+#pragma unroll
+                for (int j = 0; j < kClosureU; j++) ew[j] = k0 + j < ke ? g.edges[k0 + j] : make_int2(0, 0);
+#pragma unroll
+                for (int j = 0; j < kClosureU; j++) xu[j] = g.val[ew[j].x];
+#pragma unroll
+                for (int j = 0; j < kClosureU; j++) {
+                    if (k0 + j >= ke) break;
+                    if (xu[j] == EGS_TOP) {              // ends the path: -infinity for the game
+                        any = true;
+                        if (owner1) best = LLONG_MIN;
+                    } else if (in_d_x(g, ew[j].x, xu[j], B, last)) {
+                        any = true;
+                        const long long c = scale * ew[j].y + 1 + scale * ((long long)g.mg - xu[j]);
+                        best = owner1 ? min(best, c) : max(best, c);
+                    } else {
+                        all = false;
+                    }
                 }
             }
             if (owner1 ? !any : !all) {
@@ -190,7 +202,6 @@ __global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, lon
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[1] = 0;
 }
 
-constexpr int kBigU = 4;   // edges in flight per lane in the workgroup-per-vertex kernels
 
 // k_closure for the vertices above big_deg (launched after it), one workgroup each: the same
 // closure test and first pass; changes set their bit with an atomic OR, since k_closure has
