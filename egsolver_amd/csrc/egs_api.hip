@@ -547,7 +547,8 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
                 s->h_probe[kPinForce] = (int)s->next_round;
                 HIP_TRY(hipMemcpyAsync(s->d_ring + 11, s->h_probe + kPinForce, sizeof(int), hipMemcpyHostToDevice,
                                        s->stream));
-                s->cert_gap = s->cert_every;
+                // the spacing only grows (doubled by failures): a long run that keeps
+                // proving small sets does not pay an attempt every few rounds
                 batch = 1;        // the fixpoint often follows at once: probe again early
                 nb = 1;
             } else {

@@ -226,7 +226,8 @@ typedef struct egs_serve_options {
     int32_t lanes_per_edge_x16; /* 0 = 2 */
     int32_t group_min, group_max;
     int32_t slice_sweeps;       /* cap of complete sweeps per game per slice; 0 = none */
-    int32_t certify_every;      /* 0 = 4, < 0 = never */
+    int32_t certify_every;      /* sweeps (small games) or rounds (large route) before proof
+                                   attempts; 0 = 4 sweeps / 6 rounds, < 0 = never */
     int32_t cert_max_passes;    /* 0 = 1024 */
     int32_t slice_kedges;       /* edge budget of each workgroup per slice, in thousands of
                                    edges (bounds the slice time; sweeps resume part-way);
@@ -244,7 +245,7 @@ int64_t egs_server_free_bytes(egs_server *s);
 /* Games with at least large_n vertices take the large route: wait() solves them with the
  * device-wide single-game solver, large_workers at a time (host threads, each with its
  * own solver and stream), beside the running slice, instead of one workgroup each.
- * large_n 0 = default (65536); large_workers 0 = default (2), at most 16. */
+ * large_n 0 = default (65536); large_workers 0 = default (3), at most 16. */
 int egs_server_set_route(egs_server *s, int32_t large_n, int32_t large_workers);
 int egs_server_free_slots(egs_server *s);
 /* Returns a ticket >= 0, or EGS_E_CAPACITY when the game does not fit right now. */
