@@ -269,7 +269,9 @@ __global__ __launch_bounds__(256) void k_init(DevGame g) {
         if (v < g.n) {
             const int s = g.ptr[v], e = g.ptr[v + 1];
             bool all_neg = true, any_neg = false;
-            if (e - s <= g.big_deg) {             // above big_deg: k_init_big
+            if (e - s <= g.big_deg && g.phases > 1) {
+                act = g.stamp[v] != 0;            // decided by k_slice_count
+            } else if (e - s <= g.big_deg) {      // above big_deg: k_init_big
                 for (int k0 = s; k0 < e; k0 += 4) {
                     int w[4];
 #pragma unroll
@@ -572,17 +574,23 @@ __global__ __launch_bounds__(256) void k_slice_count(DevGame g, int *__restrict_
             const int s = g.ptr[v], e = g.ptr[v + 1];
             big = e - s > g.big_deg;
             if (!big) {
+                bool all_neg = true, any_neg = false;
                 for (int k0 = s; k0 < e; k0 += kSliceU) {
-                    int u[kSliceU];
+                    int2 ew[kSliceU];
 #pragma unroll
-                    for (int j = 0; j < kSliceU; j++) u[j] = k0 + j < e ? g.edges[k0 + j].x : -1;
+                    for (int j = 0; j < kSliceU; j++) ew[j] = k0 + j < e ? g.edges[k0 + j] : make_int2(-1, 0);
 #pragma unroll
                     for (int j = 0; j < kSliceU; j++) {
-                        const int sl = u[j] < 0 ? -1 : slice_of(u[j], g.slice_len);
+                        if (ew[j].x < 0) break;
+                        const int sl = slice_of(ew[j].x, g.slice_len);
 #pragma unroll
                         for (int q = 0; q < kPhases; q++) c[q] += sl == q;
+                        all_neg &= ew[j].y < 0;
+                        any_neg |= ew[j].y < 0;
                     }
                 }
+                // the initial activity of k_init, decided here in the same pass over the edges
+                g.stamp[v] = ((v < g.n0) ? all_neg : any_neg) ? 1 : 0;
             }
 #pragma unroll
             for (int q = 0; q < kPhases; q++) cnt[(size_t)q * g.n + v] = c[q];
