@@ -444,8 +444,9 @@ hipError_t launch_abs_wmax(const DevGame &g, int *out, hipStream_t st) {
     return hipGetLastError();
 }
 
-// Admissible thresholds, smallest first (host side): B_j = W * 2^j with an empty window,
-// some non-TOP value above it and B_j <= MG - W.  Returns how many were written.
+// Admissible thresholds (host side): B_j = W * 2^j with some non-TOP value above it and
+// B_j <= MG - W; first those with an empty window, smallest first, then the smallest whose
+// window holds at most max(64, 1/256 of the values above).  Returns how many were written.
 int gap_candidates(const int *win, const int *flo, int wmax, int mg, int64_t *out, int max_out) {
     int64_t suffix[kBins + 1];
     suffix[kBins] = 0;
@@ -455,6 +456,16 @@ int gap_candidates(const int *win, const int *flo, int wmax, int mg, int64_t *ou
         const int64_t B = (int64_t)wmax << j;
         if (B > (int64_t)mg - wmax) break;
         if (win[j] == 0 && suffix[j] > 0) out[cnt++] = B;
+    }
+    // Then the smallest threshold whose window holds only a few stragglers: the members
+    // whose edges reach them break closure and are pruned, and the rest can still be proved.
+    for (int j = 1; j < 40 && cnt < max_out; j++) {
+        const int64_t B = (int64_t)wmax << j;
+        if (B > (int64_t)mg - wmax) break;
+        if (win[j] > 0 && suffix[j] > 0 && win[j] <= std::max<int64_t>(64, suffix[j] / 256)) {
+            out[cnt++] = B;
+            break;
+        }
     }
     return cnt;
 }

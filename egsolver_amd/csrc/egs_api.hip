@@ -73,7 +73,7 @@ constexpr int kProbeWant = 100;   // offset in a probe slot: ring[13] (rounds th
 constexpr int kSparseWantRounds = 6;
 constexpr int kCertFirstCap = 64;     // longest-path passes per candidate at the first attempt
 constexpr int kCertCandidates = 8;    // gap thresholds tried per attempt (plus the last-round rise set)
-constexpr int kCertEvery = 12;        // default rounds before the first proof attempt
+constexpr int kCertEvery = 8;         // default rounds before the first proof attempt
 
 int pow2ceil(int x) {
     int p = 1;

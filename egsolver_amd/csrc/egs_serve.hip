@@ -122,7 +122,7 @@ struct LargeWorker {
 };
 
 constexpr int kSliceKedges = 2048;   // default edge budget per workgroup per slice, in thousands
-constexpr int kLargeCertifyEvery = 6;   // rounds before the first proof attempt on the large route
+constexpr int kLargeCertifyEvery = 10;  // rounds before the first proof attempt on the large route
 
 // Arena bytes of a small-route game: offsets, packed edges, values, rise stamps, potentials.
 size_t footprint(int n, int64_t m, int wmax) {
@@ -357,8 +357,8 @@ int solve_large(egs_server *s, LargeWorker &wk, int ticket, const egs_serve_opti
                       : egs_solver_load(wk.solver, EGS_MEM_DEVICE, sl.n, sl.n0, sl.m, sl.mg, (const int32_t *)base,
                                         (const int32_t *)(base + o_e), (const int32_t *)(base + o_w));
     egs_options eo{};
-    // auto: 6 rounds for these mid-size games (65K..500K vertices of the serving mix: 0.70 ms
-    // per solve against 0.80 with the single-game default of 12)
+    // auto: 10 rounds for these mid-size games (65K..500K vertices of the serving mix: 0.668 ms
+    // per solve against 0.684 with the single-game default of 8)
     eo.certify_every = o->certify_every != 0 ? o->certify_every : kLargeCertifyEvery;
     eo.cert_max_passes = o->cert_max_passes;
     eo.max_phases = -1;

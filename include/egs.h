@@ -48,7 +48,7 @@ typedef struct egs_options {
                                0 = auto (12), > 100 = never sweep */
     int32_t certify_every;  /* rounds between attempts to prove climbing vertices TOP
                                (exact; attempts back off while they fail);
-                               0 = auto (12), < 0 = never */
+                               0 = auto (8), < 0 = never */
     int32_t cert_max_passes;/* ceiling of the longest-path passes per proof candidate
                                (the cap starts at 64 and doubles while candidates run
                                out of passes); 0 = auto (1024) */
@@ -227,7 +227,7 @@ typedef struct egs_serve_options {
     int32_t group_min, group_max;
     int32_t slice_sweeps;       /* cap of complete sweeps per game per slice; 0 = none */
     int32_t certify_every;      /* sweeps (small games) or rounds (large route) before proof
-                                   attempts; 0 = 4 sweeps / 6 rounds, < 0 = never */
+                                   attempts; 0 = 4 sweeps / 10 rounds, < 0 = never */
     int32_t cert_max_passes;    /* 0 = 1024 */
     int32_t slice_kedges;       /* edge budget of each workgroup per slice, in thousands of
                                    edges (bounds the slice time; sweeps resume part-way);
