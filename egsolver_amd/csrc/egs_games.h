@@ -152,6 +152,8 @@ __device__ __forceinline__ int game_sweep_part(const GameDesc &b, const GameVals
 // sets D, tried in turn until one yields a non-empty proven set:
 //   * every admissible gap threshold B, smallest first: D = {v : B < f(v) < TOP};
 //   * the vertices that rose in the last sweep (`last`): D = {v : rise(v) == last, f(v) < TOP}.
+// When no window is empty, the smallest threshold whose window holds only a few values
+// takes the place of the gap thresholds.
 // Check 1 (closure): an owner-0 vertex of D has every successor in D or TOP, an owner-1
 // vertex at least one; vertices that break it are removed until it holds (the greatest
 // closed subset), during the passes of check 2: the longest-path game on D with w' = (n+1) w + 1,
@@ -196,13 +198,18 @@ __device__ int game_certify(const GameDesc &b, const GameVals &V, int *dp, int l
     __syncthreads();
     if (threadIdx.x == 0) {
         // bit j set: B_j = W * 2^j has an empty window, non-TOP values above it, B_j <= MG - W
-        unsigned long long mask = 0;
+        unsigned long long mask = 0, near = 0;
         int64_t above = 0;                    // #{non-TOP values > W * 2^j} while j descends
         for (int k = 63; k >= 40; k--) above += flo[k];
         for (int j = 39; j >= 1; j--) {
             above += flo[j];
-            if (((int64_t)wmax << j) <= (int64_t)b.mg - wmax && win[j] == 0 && above > 0) mask |= 1ull << j;
+            if (((int64_t)wmax << j) > (int64_t)b.mg - wmax || above == 0) continue;
+            if (win[j] == 0) mask |= 1ull << j;
+            else if (win[j] <= max((int64_t)64, above / 256)) near = 1ull << j;   // ends as the smallest
         }
+        // no empty window: the smallest threshold whose window holds a few stragglers (the
+        // members whose edges reach them are pruned)
+        if (mask == 0) mask = near;
         unsigned long long keep = 0;          // the lowest max_cands thresholds
         for (int k = 0; k < max_cands && mask; k++) {
             keep |= mask & (~mask + 1);
