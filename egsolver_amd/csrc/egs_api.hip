@@ -99,6 +99,7 @@ struct egs_solver {
     int sub_cap = 0;
     int *d_sigma = nullptr, *d_hist = nullptr, *d_bfcnt = nullptr;   // d_sigma: rise stamps
     unsigned *d_bits = nullptr;     // three bitmaps of raised potentials (proof passes)
+    int *d_hacc = nullptr, *d_hcnt = nullptr;   // shared lifts of the vertices above huge_deg
     int *d_sptr = nullptr, *d_acc = nullptr, *d_stile = nullptr;    // phased dense rounds (large n_cap)
     int2 *d_sedges = nullptr;
     unsigned long long *d_bigmask = nullptr;
@@ -232,7 +233,7 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
            o_tile = take(4 * (size_t)tiles), o_fill = take(4 * n1),
            o_stats = take(8 * 4 * (size_t)egs::kStatSlots), o_statsum = take(8 * 8),
            o_sigma = take(4 * n1), o_h = take(8 * n1), o_hist = take(4 * 256), o_bfcnt = take(4 * 64),
-           o_bits = take(4 * 3 * (n1 / 32 + n1 / 1024 + 4));
+           o_bits = take(4 * 3 * (n1 / 32 + n1 / 1024 + 4)), o_hacc = take(4 * n1), o_hcnt = take(4 * n1);
     // phased dense rounds: slice offsets, slice-major edges, partial keys, scan tiles
     const bool phased = n_cap >= egs::kPhaseMinN;
     const size_t pn1 = (size_t)egs::kPhases * n_cap + 1;
@@ -271,6 +272,8 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
     s->d_hist = (int *)(b + o_hist);
     s->d_bfcnt = (int *)(b + o_bfcnt);
     s->d_bits = (unsigned *)(b + o_bits);
+    s->d_hacc = (int *)(b + o_hacc);
+    s->d_hcnt = (int *)(b + o_hcnt);
     if (phased) {
         s->d_sptr = (int *)(b + o_sptr);
         s->d_sedges = (int2 *)(b + o_sedges);
@@ -421,7 +424,7 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     sh.use_wave = s->max_outdeg > g.big_deg;
     sh.use_block = s->max_outdeg > g.huge_deg;
     sh.wave_grid = 1024;
-    sh.block_grid = 512;
+    sh.block_grid = 1024;
     s->batch_rounds = o->batch_rounds > 0 ? o->batch_rounds : 32;
     s->phase_cap = egs_max_loop_c(g.m, g.n, g.mg, o->max_phases);
     s->stop_flag = o->stop_flag;
@@ -436,6 +439,8 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     g.sedges = s->d_sedges;
     g.sshift = o->slices == 1 ? 0 : egs::pack_shift(g.n, s->wmax);   // 32-bit slice edges when they fit
     g.acc = s->d_acc;
+    g.hacc = s->d_hacc;
+    g.hcnt = s->d_hcnt;
     g.bigmask = s->d_bigmask;
     // one lane per vertex up to two edges per slice on average; eight 256-thread workgroups
     // per CU (a sweep on the 2^22-vertex game: 268 us per round against 282 with two lanes
@@ -471,6 +476,10 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     s->csc_built = false;
     HIP_TRY(hipMemsetAsync(s->d_ring + 12, 0, 2 * sizeof(int), s->stream));
     if (s->g.dense_min > s->g.n) HIP_TRY(build_csc(s));     // dense sweeps switched off
+    if (s->sh.use_block) {           // shared lifts of the vertices above huge_deg start empty
+        HIP_TRY(hipMemsetD32Async((hipDeviceptr_t)s->d_hacc, EGS_TOP, (size_t)s->g.n, s->stream));
+        HIP_TRY(hipMemsetAsync(s->d_hcnt, 0, sizeof(int) * (size_t)s->g.n, s->stream));
+    }
     HIP_TRY(egs::launch_classify(s->g, s->d_ring + 9, s->stream));
     HIP_TRY(egs::launch_slices(s->g, s->d_stile, s->stream));
     HIP_TRY(egs::launch_init(s->g, s->stream));

@@ -55,6 +55,8 @@ struct DevGame {
     int sshift;
     const unsigned long long *bigmask;   // bit v: vertex v is above big_deg (lifted by the wave/block kernels)
     int *acc;             // n partial keys carried between phases
+    int *hacc, *hcnt;     // n each: partial keys and finished chunks of the vertices above
+                          // huge_deg (TOP and 0 between rounds)
 };
 
 constexpr int kPhases = 4;                 // successor slices of a phased dense round

@@ -663,12 +663,19 @@ __global__ __launch_bounds__(256) void k_lift_wave(DevGame g, int r) {
     flush_stats(g, acc);
 }
 
-// One workgroup (256 lanes) per vertex: the static huge list in dense rounds, the
-// deferred list in sparse rounds.
+// Vertices above huge_deg: the static huge list in dense rounds, the deferred list in
+// sparse rounds.  A vertex is shared by up to `per` workgroups (the grid over the items):
+// each takes every per-th chunk of kHugeChunk edges, folds its minimum key into hacc[v]
+// with an atomic, and the workgroup that completes hcnt[v] applies the lift (value, rise
+// stamp, list or predecessor push) and resets both for the next round.  One workgroup
+// walking a hub of 70K edges alone made this kernel most of a skewed game's round.
+constexpr int kHugeChunk = 2048;
+
 __global__ __launch_bounds__(256) void k_lift_block(DevGame g, int r) {
     __shared__ int sbuf[kLdsCap];
     __shared__ int scnt;
     __shared__ int wmin[4];
+    __shared__ int s_key, s_last;
     const int mode = g.mode[r % 3];
     if (mode == MODE_DONE) return;
     const bool dense = mode == MODE_DENSE;
@@ -682,27 +689,55 @@ __global__ __launch_bounds__(256) void k_lift_block(DevGame g, int r) {
     LiftAcc acc;
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
-    for (int64_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+    const int per = max(1, (int)gridDim.x / cnt);        // workgroups per vertex
+    const int64_t units = (int64_t)cnt * per;
+    for (int64_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const int i = (int)(u % cnt), part = (int)(u / cnt);
         const int v = items[i];
         const int s = g.ptr[v], e = g.ptr[v + 1];
-        const int old = g.val[v];
+        const int chunks = (e - s + kHugeChunk - 1) / kHugeChunk;
+        const int contrib = min(per, chunks);
+        if (part >= contrib) continue;                   // the whole workgroup: u is shared
         const bool owner1 = v >= g.n0;
-        int key = key_over_edges<8>(g, s, e, (int)threadIdx.x, 256, owner1);
+        int key = EGS_TOP;
+        for (int c = part; c < chunks; c += contrib) {
+            const int cs = s + c * kHugeChunk, ce = min(e, cs + kHugeChunk);
+            key = min(key, key_over_edges<8>(g, cs, ce, (int)threadIdx.x, 256, owner1));
+        }
         key = egs_group_min<64>(key);
         if (lane == 0) wmin[wid] = key;
         __syncthreads();
-        key = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
-        __syncthreads();
-        const int t = egs_unkey(key, owner1);
-        const bool lifted = old < t;
         if (threadIdx.x == 0) {
-            acc.processed++;
-            acc.edges += (uint32_t)(e - s);
-            if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; }
+            key = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
+            for (int c = part; c < chunks; c += contrib) acc.edges += (uint32_t)(min(e, s + (c + 1) * kHugeChunk) - (s + c * kHugeChunk));
+            bool last = true;
+            if (contrib > 1) {
+                atomicMin(&g.hacc[v], key);
+                __threadfence();
+                last = atomicAdd(&g.hcnt[v], 1) == contrib - 1;
+                if (last) {
+                    __threadfence();
+                    key = atomicExch(&g.hacc[v], EGS_TOP);   // every contribution, and the reset
+                    g.hcnt[v] = 0;
+                }
+            }
+            s_key = key;
+            s_last = last;
         }
-        if (dense) stage_push(g, st, lifted && threadIdx.x == 0, v);
-        else push_preds<256>(g, st, v, lifted, (int)threadIdx.x, r, acc);
+        __syncthreads();
+        if (s_last) {
+            const int t = egs_unkey(s_key, owner1);
+            const int old = g.val[v];
+            const bool lifted = old < t;
+            if (threadIdx.x == 0) {
+                acc.processed++;
+                if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; }
+            }
+            if (dense) stage_push(g, st, lifted && threadIdx.x == 0, v);
+            else push_preds<256>(g, st, v, lifted, (int)threadIdx.x, r, acc);
+        }
         stage_flush(g, st, kLdsCap / 2);
+        __syncthreads();
     }
     stage_flush(g, st, 1);
     flush_stats(g, acc);

@@ -194,6 +194,27 @@ def test_pruned_proofs_and_hubs_vs_oracle(oracle):
     assert certs > 0 and pruned > 0, (certs, pruned)
 
 
+def test_split_hub_lifts_vs_oracle(oracle):
+    """Vertices above huge_deg with thousands of edges are lifted by several workgroups at
+    once (chunks of 2048 edges meeting in a per-vertex accumulator), in dense and in sparse
+    rounds: exact against the oracle."""
+    import egsolver_amd as E
+    for seed in range(3):
+        g = E.generate(E.GenParams(n=6000, seed=500 + seed, deg_kind=2, deg_lo=1, deg_hi=5999, alpha=1.5, wmax=60))
+        deg = np.diff(g.ptr)
+        _say(f"split hub lifts seed {seed} n {g.n} m {g.m} maxdeg {deg.max()} above 2048: {(deg > 2048).sum()}")
+        assert (deg > 2048).sum() >= 2             # at least two chunks each
+        ref = lfp(oracle, to_oracle_game(g))
+        with E.Solver(g.n, g.m) as s:
+            s.load(g)
+            for opt in (E.options(huge_deg=256), E.options(huge_deg=256, dense_pct=1), E.options(huge_deg=256, certify_every=-1),
+                        E.options(huge_deg=1024, dense_pct=101)):
+                s.prepare(opt)
+                conv, st = s.iterate()
+                assert conv
+                assert np.array_equal(s.values(), ref), (seed, opt.huge_deg, opt.dense_pct)
+
+
 def test_violations_count_with_hubs(oracle):
     """The device fixpoint check (wide vertices shared by the wave) counts the same
     violations as the oracle on the partial values of a power-law game."""
