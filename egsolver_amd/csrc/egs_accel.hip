@@ -338,15 +338,15 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
     block_add(removed, &flags[0]);
 }
 
-// Pass p for the vertices above big_deg (launched after k_bf_pass for the same p), one
-// workgroup each, with the same rule and updates.
+// Pass p for the vertices of the big list (big_deg < deg <= huge_deg; launched after
+// k_bf_pass for the same p), one workgroup each, with the same rule and updates.
 __global__ __launch_bounds__(256) void k_bf_big(DevGame g, long long *__restrict__ h, long long bound,
                                                 const unsigned *__restrict__ prev, unsigned *__restrict__ cur,
                                                 int p, int *__restrict__ cnt, int *__restrict__ flags) {
     __shared__ long long sm[4];
     if (cnt[(p - 1) % kCntSlots] == 0 || flags[1]) return;
     const long long scale = (long long)g.n + 1;
-    const int nb = g.ring[9], items = nb + g.ring[10];
+    const int nb = g.ring[9], items = nb;               // the huge list: k_bf_huge
     int local = 0;
     for (int64_t i = blockIdx.x; i < items; i += gridDim.x) {
         const int v = egs_big_item(g.sbig, g.shuge, nb, items - 1 - i);   // huge vertices first
@@ -401,6 +401,116 @@ __global__ __launch_bounds__(256) void k_bf_big(DevGame g, long long *__restrict
         }
     }
     if (threadIdx.x == 0 && local) atomicAdd(&cnt[p % kCntSlots], local);
+}
+
+// Pass p for the vertices above huge_deg, each shared by up to `per` workgroups: a
+// workgroup takes every per-th chunk of kHugeChunkBf edges and folds into the vertex's
+// accumulators whether a successor changed, whether an edge leaves D, and the min (owner 1)
+// or max (owner 0) candidate; the workgroup that completes hcnt[v] applies the update as
+// k_bf_pass does (only when a successor changed) and resets the accumulators.  All
+// contributors read h[v] before any of them counts in, so they agree on membership.
+constexpr int kHugeChunkBf = 2048;
+
+__global__ __launch_bounds__(256) void k_bf_huge(DevGame g, long long *__restrict__ h, long long bound,
+                                                 const unsigned *__restrict__ prev, unsigned *__restrict__ cur,
+                                                 int p, int *__restrict__ cnt, int *__restrict__ flags) {
+    __shared__ long long sm[4];
+    __shared__ long long s_best;
+    __shared__ int s_last, s_flag;
+    if (cnt[(p - 1) % kCntSlots] == 0 || flags[1]) return;
+    const int nh = g.ring[10];
+    if (nh == 0) return;
+    const long long scale = (long long)g.n + 1;
+    const int per = max(1, (int)gridDim.x / nh);
+    const int64_t units = (int64_t)nh * per;
+    int local = 0, removed = 0;
+    for (int64_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const int i = (int)(u % nh), part = (int)(u / nh);
+        const int v = g.shuge[nh - 1 - i];                 // largest first
+        const long long hv = h[v];
+        if (hv < 0) continue;                               // the whole workgroup
+        const int kb = g.ptr[v], ke = g.ptr[v + 1];
+        const int chunks = (ke - kb + kHugeChunkBf - 1) / kHugeChunkBf;
+        const int contrib = min(per, chunks);
+        if (part >= contrib) continue;
+        const bool owner1 = v >= g.n0;
+        long long best = owner1 ? LLONG_MAX : LLONG_MIN;
+        bool need = false, out = false;
+        for (int c = part; c < chunks; c += contrib) {
+            const int cs = kb + c * kHugeChunkBf, ce = min(ke, cs + kHugeChunkBf);
+            for (int k0 = cs + (int)threadIdx.x; k0 < ce; k0 += kBigU * 256) {
+                int2 ew[kBigU];
+                long long hu[kBigU];
+#pragma unroll
+                for (int j = 0; j < kBigU; j++) ew[j] = k0 + 256 * j < ce ? g.edges[k0 + 256 * j] : make_int2(0, 0);
+#pragma unroll
+                for (int j = 0; j < kBigU; j++) hu[j] = h[ew[j].x];
+#pragma unroll
+                for (int j = 0; j < kBigU; j++) {
+                    if (k0 + 256 * j >= ce) break;
+                    need |= bm_test(prev, g.n, ew[j].x);
+                    if (hu[j] == kHOut) { out = true; continue; }
+                    const long long c2 = hu[j] == kHTop ? LLONG_MIN : scale * ew[j].y + 1 + hu[j];
+                    best = owner1 ? min(best, c2) : max(best, c2);
+                }
+            }
+        }
+        need = __syncthreads_or(need);
+        out = __syncthreads_or(out);
+        best = egs_block_minmax_ll(best, owner1, sm);
+        if (threadIdx.x == 0) {
+            bool last = true;
+            int f = (need ? 1 : 0) | (out ? 2 : 0);
+            if (contrib > 1) {
+                if (f) atomicOr(&g.hflag[v], f);
+                if (owner1) atomicMin(&g.hbest[v], best);
+                else atomicMax(&g.hbest[v], best);
+                __threadfence();
+                last = atomicAdd(&g.hcnt[v], 1) == contrib - 1;
+                if (last) {
+                    __threadfence();
+                    f = atomicExch(&g.hflag[v], 0);
+                    best = atomicExch((unsigned long long *)&g.hbest[v],
+                                      (unsigned long long)(owner1 ? LLONG_MAX : LLONG_MIN));
+                    g.hcnt[v] = 0;
+                }
+            }
+            s_last = last;
+            s_flag = f;
+            s_best = best;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && s_last && (s_flag & 1)) {
+            const long long b = s_best;
+            if (owner1 ? b == LLONG_MAX : (s_flag & 2) != 0) {
+                h[v] = kHOut;
+                bm_set(cur, g.n, v);
+                local++;
+                removed++;
+            } else if (b > hv) {
+                h[v] = b;
+                bm_set(cur, g.n, v);
+                local++;
+                if (b > bound) atomicOr(&flags[1], 1);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && local) atomicAdd(&cnt[p % kCntSlots], local);
+    if (threadIdx.x == 0 && removed) atomicAdd(&flags[0], removed);
+}
+
+// Starting values of the accumulators of k_bf_huge: the identity of each owner's fold.
+__global__ __launch_bounds__(256) void k_hbest_reset(DevGame g) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
+        g.hbest[v] = v >= g.n0 ? LLONG_MAX : LLONG_MIN;
+        g.hflag[v] = 0;
+    }
+}
+
+hipError_t launch_hbest_reset(const DevGame &g, hipStream_t st) {
+    hipLaunchKernelGGL(k_hbest_reset, dim3(grid_for(g.n, 256, 2048)), dim3(256), 0, st, g);
+    return hipGetLastError();
 }
 
 // D is where h >= 0 (set by k_closure for this candidate).
@@ -538,9 +648,12 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
             for (int p = queued; p < upto; p++) {
                 launch_bf_pass(G, bf_grid, st, g, cb.h, bound, bm[(p - 1) % 3], bm[p % 3], bm[(p + 1) % 3], p, cb.bfcnt,
                                flags);
-                if (has_big)
+                if (has_big) {
                     hipLaunchKernelGGL(k_bf_big, dim3(kBigGrid), dim3(256), 0, st, g, cb.h, bound, bm[(p - 1) % 3],
                                        bm[p % 3], p, cb.bfcnt, flags);
+                    hipLaunchKernelGGL(k_bf_huge, dim3(2 * kBigGrid), dim3(256), 0, st, g, cb.h, bound,
+                                       bm[(p - 1) % 3], bm[p % 3], p, cb.bfcnt, flags);
+                }
             }
             if (hipMemcpyAsync(fl, flags, sizeof(int) * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
             if (hipMemcpyAsync(cnt, cb.bfcnt, sizeof(int) * kCntSlots, hipMemcpyDeviceToHost, st) != hipSuccess)

@@ -100,6 +100,8 @@ struct egs_solver {
     int *d_sigma = nullptr, *d_hist = nullptr, *d_bfcnt = nullptr;   // d_sigma: rise stamps
     unsigned *d_bits = nullptr;     // three bitmaps of raised potentials (proof passes)
     int *d_hacc = nullptr, *d_hcnt = nullptr;   // shared lifts of the vertices above huge_deg
+    long long *d_hbest = nullptr;               // shared proof passes of the same vertices
+    int *d_hflag = nullptr;
     int *d_sptr = nullptr, *d_acc = nullptr, *d_stile = nullptr;    // phased dense rounds (large n_cap)
     int2 *d_sedges = nullptr;
     unsigned long long *d_bigmask = nullptr;
@@ -233,7 +235,8 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
            o_tile = take(4 * (size_t)tiles), o_fill = take(4 * n1),
            o_stats = take(8 * 4 * (size_t)egs::kStatSlots), o_statsum = take(8 * 8),
            o_sigma = take(4 * n1), o_h = take(8 * n1), o_hist = take(4 * 256), o_bfcnt = take(4 * 64),
-           o_bits = take(4 * 3 * (n1 / 32 + n1 / 1024 + 4)), o_hacc = take(4 * n1), o_hcnt = take(4 * n1);
+           o_bits = take(4 * 3 * (n1 / 32 + n1 / 1024 + 4)), o_hacc = take(4 * n1), o_hcnt = take(4 * n1),
+           o_hbest = take(8 * n1), o_hflag = take(4 * n1);
     // phased dense rounds: slice offsets, slice-major edges, partial keys, scan tiles
     const bool phased = n_cap >= egs::kPhaseMinN;
     const size_t pn1 = (size_t)egs::kPhases * n_cap + 1;
@@ -274,6 +277,8 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
     s->d_bits = (unsigned *)(b + o_bits);
     s->d_hacc = (int *)(b + o_hacc);
     s->d_hcnt = (int *)(b + o_hcnt);
+    s->d_hbest = (long long *)(b + o_hbest);
+    s->d_hflag = (int *)(b + o_hflag);
     if (phased) {
         s->d_sptr = (int *)(b + o_sptr);
         s->d_sedges = (int2 *)(b + o_sedges);
@@ -441,6 +446,8 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     g.acc = s->d_acc;
     g.hacc = s->d_hacc;
     g.hcnt = s->d_hcnt;
+    g.hbest = s->d_hbest;
+    g.hflag = s->d_hflag;
     g.bigmask = s->d_bigmask;
     // one lane per vertex up to two edges per slice on average; eight 256-thread workgroups
     // per CU (a sweep on the 2^22-vertex game: 268 us per round against 282 with two lanes
@@ -479,6 +486,7 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     if (s->sh.use_block) {           // shared lifts of the vertices above huge_deg start empty
         HIP_TRY(hipMemsetD32Async((hipDeviceptr_t)s->d_hacc, EGS_TOP, (size_t)s->g.n, s->stream));
         HIP_TRY(hipMemsetAsync(s->d_hcnt, 0, sizeof(int) * (size_t)s->g.n, s->stream));
+        HIP_TRY(egs::launch_hbest_reset(s->g, s->stream));
     }
     HIP_TRY(egs::launch_classify(s->g, s->d_ring + 9, s->stream));
     HIP_TRY(egs::launch_slices(s->g, s->d_stile, s->stream));

@@ -57,6 +57,9 @@ struct DevGame {
     int *acc;             // n partial keys carried between phases
     int *hacc, *hcnt;     // n each: partial keys and finished chunks of the vertices above
                           // huge_deg (TOP and 0 between rounds)
+    long long *hbest;     // n: partial longest-path candidates of the vertices above huge_deg
+                          // in proof passes (each owner's identity between passes)
+    int *hflag;           // n: bit 0 a successor changed, bit 1 an edge leaves D (0 between passes)
 };
 
 constexpr int kPhases = 4;                 // successor slices of a phased dense round
@@ -85,6 +88,7 @@ hipError_t launch_sum_stats(const DevGame &g, int nslots, unsigned long long *ou
 hipError_t launch_degree_stats(const int *ptr, int n, int *out, hipStream_t st);
 hipError_t launch_violations(const DevGame &g, unsigned long long *out, hipStream_t st);
 hipError_t launch_winning(const int *val, int n, unsigned *out, hipStream_t st);
+hipError_t launch_hbest_reset(const DevGame &g, hipStream_t st);
 int grid_for(int64_t work, int threads, int cap);
 
 // Certified TOP detection (egs_accel.hip).
