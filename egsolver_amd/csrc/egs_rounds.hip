@@ -211,7 +211,7 @@ template <int G>
 __device__ __forceinline__ int lift_value(const DevGame &g, int s, int e, int gl, bool owner1) {
     int key = EGS_TOP;
     if constexpr (G == 64) {
-        key = key_over_edges<4>(g, s, e, gl, 64, owner1);
+        key = key_over_edges<8>(g, s, e, gl, 64, owner1);
     } else {
         for (int k = s + gl; k < e; k += G) {
             const int2 ew = g.edges[k];
