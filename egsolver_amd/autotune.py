@@ -55,8 +55,7 @@ def stratified_sample(games, k, seed=0):
     return [games[order[i]] for i in pick]
 
 
-def tune_batch(games, sample=48, threads=(256, 512, 1024), lds=(8192, 16384, 38000), per_cu=(1, 2, 4),
-               lanes_x16=(4, 8, 16), device=0):
+def tune_batch(games, sample=48, threads=(512, 1024), lds=(0, 20000), per_cu=(1, 2), lanes_x16=(1, 2, 4), device=0):
     sub = stratified_sample(games, sample)
     table = []
     with Batch(device) as b:
@@ -64,8 +63,8 @@ def tune_batch(games, sample=48, threads=(256, 512, 1024), lds=(8192, 16384, 380
             b.add(g)
         b.upload()
         for th, ld, pc, lx in itertools.product(threads, lds, per_cu, lanes_x16):
-            if th == 1024 and pc > 2:
-                continue
+            if pc > 1 and ld == 0:
+                continue          # lds 0 = the whole LDS share: one workgroup per CU
             opt = batch_options(threads=th, lds_ints=ld, blocks_per_cu=pc, lanes_per_edge_x16=lx)
             try:
                 st = b.solve(opt)
