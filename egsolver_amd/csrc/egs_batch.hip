@@ -87,6 +87,8 @@ struct egs_batch {
     // the other one out
     int32_t *h_stage[2] = {nullptr, nullptr};
     hipEvent_t ev_stage[2] = {nullptr, nullptr};
+    int32_t *h_values = nullptr;   // pinned copy of every value (egs_batch_read_all_pinned)
+    int64_t h_values_n = 0;
 };
 
 constexpr size_t kStageInts = (size_t)2 << 20;   // 8 MB per read-back chunk
@@ -137,6 +139,7 @@ int egs_batch_destroy(egs_batch *b) {
         if (b->ev_stage[k]) (void)hipEventDestroy(b->ev_stage[k]);
         if (b->h_stage[k]) (void)hipHostFree(b->h_stage[k]);
     }
+    if (b->h_values) (void)hipHostFree(b->h_values);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return EGS_OK;
@@ -345,6 +348,23 @@ int egs_batch_read_all(egs_batch *b, int32_t *out) {
             egs_pk_copy(out + (c - 1) * (int64_t)kStageInts, b->h_stage[(c - 1) & 1], 4 * len(c - 1));
         }
     }
+    return EGS_OK;
+}
+
+int egs_batch_read_all_pinned(egs_batch *b, const int32_t **values) {
+    if (!b || !b->uploaded || !values) return bfail(EGS_E_STATE, "nothing uploaded");
+    BHIP(hipSetDevice(b->device));
+    if (b->h_values_n < b->total_n) {
+        if (b->h_values) (void)hipHostFree(b->h_values);
+        b->h_values = nullptr;
+        b->h_values_n = 0;
+        if (hipHostMalloc((void **)&b->h_values, 4 * (size_t)b->total_n, hipHostMallocDefault) != hipSuccess)
+            return bfail(EGS_E_NOMEM, "hipHostMalloc of %lld values failed", (long long)b->total_n);
+        b->h_values_n = b->total_n;
+    }
+    BHIP(hipMemcpyAsync(b->h_values, b->d_values, 4 * (size_t)b->total_n, hipMemcpyDeviceToHost, b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
+    *values = b->h_values;
     return EGS_OK;
 }
 

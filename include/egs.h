@@ -205,6 +205,10 @@ int egs_batch_upload(egs_batch *b);
 int egs_batch_solve(egs_batch *b, const egs_batch_options *opt, egs_batch_stats *st);
 int egs_batch_read_values(egs_batch *b, int game, int32_t *out);
 int egs_batch_read_all(egs_batch *b, int32_t *out);
+/* No reference counterpart.  Every game's values, concatenated in game order, in a pinned
+ * buffer the batch owns: one device-to-host copy with no host copy after it.  *values
+ * stays valid until the next read or solve of this batch and its destruction. */
+int egs_batch_read_all_pinned(egs_batch *b, const int32_t **values);
 int egs_batch_game_stats(egs_batch *b, int game, int64_t *sweeps, int64_t *edges, int32_t *converged);
 const char *egs_batch_last_error(void);
 

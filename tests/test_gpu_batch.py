@@ -49,6 +49,8 @@ def test_batch_matches_oracle_and_alone(oracle):
             vals = b.all_values()
             for i, (g, ref) in enumerate(zip(games, refs)):
                 assert np.array_equal(vals[i], ref), (i, g.n, np.count_nonzero(vals[i] != ref))
+            view = b.values_view()           # the pinned read-back: same values
+            assert all(np.array_equal(view[i], vals[i]) for i in range(len(games)))
     for i in (0, 7, 19, 33):
         alone, _ = E.solve(games[i])
         assert np.array_equal(alone, refs[i])
