@@ -145,7 +145,7 @@ class Server:
 
     def set_route(self, large_n=0, large_workers=0):
         """Games with at least `large_n` vertices go to the device-wide route (0 = default,
-        65536), solved `large_workers` at a time (0 = default, 2)."""
+        65536), solved `large_workers` at a time (0 = default, 3)."""
         _check(N.egs().egs_server_set_route(self.h, int(large_n), int(large_workers)), "egs_server_set_route")
 
 
@@ -201,11 +201,10 @@ def run_served(requests, arena_bytes, opt=None, max_games=4096, keep_values=Fals
             # admissions while the slice runs overlap its execution
             admit(max_games, while_busy=True)
             done = srv.wait()
-            tf = time.perf_counter() - t0
             for t in done:
                 r = running.pop(t)
-                vals, _ = srv.fetch(t, want_values=keep_values)
-                r.finished = tf
+                vals, _ = srv.fetch(t)           # every result is read back, as the plain path does
+                r.finished = time.perf_counter() - t0
                 if keep_values:
                     r.values = vals
         wall = time.perf_counter() - t0
