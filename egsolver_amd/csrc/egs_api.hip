@@ -102,7 +102,7 @@ struct egs_solver {
     int *d_hacc = nullptr, *d_hcnt = nullptr;   // shared lifts of the vertices above huge_deg
     long long *d_hbest = nullptr;               // shared proof passes of the same vertices
     int *d_hflag = nullptr;
-    int *d_sptr = nullptr, *d_acc = nullptr, *d_stile = nullptr;    // phased dense rounds (large n_cap)
+    int *d_sptr = nullptr, *d_acc = nullptr;    // phased dense rounds (large n_cap)
     int2 *d_sedges = nullptr;
     unsigned long long *d_bigmask = nullptr;
     long long *d_h = nullptr;
@@ -237,12 +237,11 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
            o_sigma = take(4 * n1), o_h = take(8 * n1), o_hist = take(4 * 256), o_bfcnt = take(4 * 64),
            o_bits = take(4 * 3 * (n1 / 32 + n1 / 1024 + 4)), o_hacc = take(4 * n1), o_hcnt = take(4 * n1),
            o_hbest = take(8 * n1), o_hflag = take(4 * n1);
-    // phased dense rounds: slice offsets, slice-major edges, partial keys, scan tiles
+    // phased dense rounds: slice offsets per tile, sliced edges, partial keys
     const bool phased = n_cap >= egs::kPhaseMinN;
-    const size_t pn1 = (size_t)egs::kPhases * n_cap + 1;
+    const size_t pn1 = ((size_t)n_cap / egs::kSliceTile + 1) * egs::kSliceStride;
     const size_t o_sptr = phased ? take(4 * pn1) : 0, o_sedges = phased ? take(8 * (size_t)m_cap) : 0,
-                 o_acc = phased ? take(4 * n1) : 0, o_stile = phased ? take(4 * (pn1 / 4096 + 2)) : 0,
-                 o_bigmask = phased ? take(8 * ((size_t)n_cap / 64 + 2)) : 0;
+                 o_acc = phased ? take(4 * n1) : 0, o_bigmask = phased ? take(8 * ((size_t)n_cap / 64 + 2)) : 0;
     s->block_bytes = off;
     if (hipMalloc(&s->block, off) != hipSuccess) {
         egs_solver_destroy(s);
@@ -283,7 +282,6 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
         s->d_sptr = (int *)(b + o_sptr);
         s->d_sedges = (int2 *)(b + o_sedges);
         s->d_acc = (int *)(b + o_acc);
-        s->d_stile = (int *)(b + o_stile);
         s->d_bigmask = (unsigned long long *)(b + o_bigmask);
     }
     if (hipHostMalloc((void **)&s->h_probe, 8192, hipHostMallocDefault) != hipSuccess) {
@@ -489,7 +487,7 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
         HIP_TRY(egs::launch_hbest_reset(s->g, s->stream));
     }
     HIP_TRY(egs::launch_classify(s->g, s->d_ring + 9, s->stream));
-    HIP_TRY(egs::launch_slices(s->g, s->d_stile, s->stream));
+    HIP_TRY(egs::launch_slices(s->g, s->stream));
     HIP_TRY(egs::launch_init(s->g, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_ring + 11, 0, sizeof(int), s->stream));
     HIP_TRY(hipEventRecord(s->ev_b, s->stream));

@@ -56,6 +56,32 @@ __device__ __forceinline__ int egs_wave_sum(int x) {
     return __builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 32);
 }
 
+// Exclusive scan over a workgroup (a multiple of 64 threads): wave scans with shuffles,
+// wave totals combined through lds_warp (one int per wave).  Returns this thread's
+// exclusive prefix; total = the sum over the workgroup.  Every thread must call it.
+__device__ __forceinline__ int egs_block_exclusive_scan(int x, int *lds_warp, int &total) {
+    // wave-level inclusive scan with shuffles, then wave totals combined through LDS.
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    int inc = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int y = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += y;
+    }
+    if (lane == 63) lds_warp[wid] = inc;
+    __syncthreads();
+    int wave_off = 0;
+    total = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+        int s = lds_warp[i];
+        if (i < wid) wave_off += s;
+        total += s;
+    }
+    __syncthreads();
+    return wave_off + inc - x;
+}
+
 __device__ __forceinline__ unsigned egs_lane() { return __lane_id(); }
 
 // Lanes below this one in the wave, as a mask.

@@ -49,9 +49,12 @@ struct DevGame {
     // successor lies in [s * slice_len, (s+1) * slice_len), so its value gathers stay in L2.
     int phases;           // 1 = one in-place sweep (body_dense)
     int slice_len;
-    const int *sptr;      // phases * n + 1 offsets: edges of (slice s, vertex v) start at sptr[s*n + v]
-    const void *sedges;   // the low-degree vertices' edges grouped by (slice, vertex): uint32
-                          // (w << sshift) | succ when sshift > 0, else int2 {succ, w}
+    // Tiles of kSliceTile vertices: tile T's low-degree edges lie in its own CSR range
+    // from ptr[T * kSliceTile], grouped by (slice, vertex); its kSliceStride offsets
+    // sptr[T * kSliceStride + s * kSliceTile + j] start (slice s, vertex T*kSliceTile + j)
+    // and the last one ends the tile, so segment i ends where i + 1 starts.
+    const int *sptr;
+    const void *sedges;   // uint32 (w << sshift) | succ when sshift > 0, else int2 {succ, w}
     int sshift;
     const unsigned long long *bigmask;   // bit v: vertex v is above big_deg (lifted by the wave/block kernels)
     int *acc;             // n partial keys carried between phases
@@ -64,6 +67,8 @@ struct DevGame {
 
 constexpr int kPhases = 4;                 // successor slices of a phased dense round
 constexpr int kPhaseMinN = 1 << 20;        // below this the value array fits the L2 anyway
+constexpr int kSliceTile = 256;            // vertices per tile of the slice layout
+constexpr int kSliceStride = kPhases * kSliceTile + 1;
 
 // Launch shape of one round.
 struct LaunchShape {
@@ -80,9 +85,8 @@ hipError_t launch_unpack(const uint32_t *packed, int shift, int2 *edges, int64_t
 hipError_t launch_transpose(const DevGame &g, int *tile_sum, int *fill, hipStream_t st);
 hipError_t launch_classify(const DevGame &g, int *counts, hipStream_t st);
 hipError_t launch_init(const DevGame &g, hipStream_t st);
-hipError_t launch_slices(const DevGame &g, int *tile_sum, hipStream_t st);
+hipError_t launch_slices(const DevGame &g, hipStream_t st);
 int pack_shift(int n, int wmax);
-void launch_scan_inplace(int *cnt, int n, int *tile_sum, hipStream_t st);
 hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int r, hipStream_t st);
 hipError_t launch_sum_stats(const DevGame &g, int nslots, unsigned long long *out, hipStream_t st);
 hipError_t launch_degree_stats(const int *ptr, int n, int *out, hipStream_t st);

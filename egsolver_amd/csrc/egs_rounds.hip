@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void k_init(DevGame g) {
             const int s = g.ptr[v], e = g.ptr[v + 1];
             bool all_neg = true, any_neg = false;
             if (e - s <= g.big_deg && g.phases > 1) {
-                act = g.stamp[v] != 0;            // decided by k_slice_count
+                act = g.stamp[v] != 0;            // decided by k_slice_build
             } else if (e - s <= g.big_deg) {      // above big_deg: k_init_big
                 for (int k0 = s; k0 < e; k0 += 4) {
                     int w[4];
@@ -504,15 +504,15 @@ __global__ __launch_bounds__(256) void k_phase(DevGame g, int r, int s) {
     LiftAcc acc;
     constexpr int ITEMS = 256 / G;
     const int gl = threadIdx.x % G;
-    const int *sp = g.sptr + (size_t)s * g.n;
     int it = 0;
     for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < g.n; t0 += (int64_t)gridDim.x * ITEMS, it++) {
         const int v = (int)(t0 + threadIdx.x / G);
         const bool valid = v < g.n;
         int b = 0, e = 0;
         if (valid) {
-            b = sp[v];
-            e = sp[v + 1];
+            const size_t i = (size_t)(v / kSliceTile) * kSliceStride + (size_t)s * kSliceTile + v % kSliceTile;
+            b = g.sptr[i];
+            e = g.sptr[i + 1];
         }
         const bool owner1 = v >= g.n0;
         int key = EGS_TOP;
@@ -550,9 +550,6 @@ __global__ __launch_bounds__(256) void k_phase(DevGame g, int r, int s) {
     flush_stats(g, acc);
 }
 
-// Slice layout of the low-degree vertices' edges: counts per (slice, vertex) into sptr,
-// an in-place exclusive scan, then each vertex places its edges (no atomics).
-// Also marks the vertices above big_deg in bigmask (one bit per vertex, from a wave ballot).
 // Slice of a successor: phases of slice_len successors each (comparisons, no division).
 __device__ __forceinline__ int slice_of(int u, int slice_len) {
     int q = 0;
@@ -561,68 +558,97 @@ __device__ __forceinline__ int slice_of(int u, int slice_len) {
     return q;
 }
 
-// Edges read ahead per thread in the slice set-up kernels (one thread per vertex).
+// Edges read ahead per thread in the slice set-up kernel (one thread per vertex).
 constexpr int kSliceU = 4;
 
-__global__ __launch_bounds__(256) void k_slice_count(DevGame g, int *__restrict__ cnt,
-                                                     unsigned long long *__restrict__ bigmask) {
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < g.n; base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t v = base + threadIdx.x;
-        bool big = false;
-        if (v < g.n) {
-            int c[kPhases] = {};
-            const int s = g.ptr[v], e = g.ptr[v + 1];
-            big = e - s > g.big_deg;
-            if (!big) {
-                bool all_neg = true, any_neg = false;
-                for (int k0 = s; k0 < e; k0 += kSliceU) {
-                    int2 ew[kSliceU];
+// Slice layout of the low-degree vertices' edges in one pass, one tile of kSliceTile
+// vertices per workgroup iteration and no global scan: each thread counts its vertex's
+// edges per slice, the workgroup scans the tile's counts in (slice, vertex) order, writes
+// the tile's kSliceStride offsets from ptr[first vertex of the tile], and each thread then
+// places its edges (read again, mostly from L2).  Vertices past n and those above big_deg
+// get empty segments.  The same pass marks the vertices above big_deg in bigmask (a wave
+// ballot) and decides k_init's initial activity of the others into stamp.
+__global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, int *__restrict__ sptr,
+                                                           void *__restrict__ sedges,
+                                                           unsigned long long *__restrict__ bigmask) {
+    __shared__ int s_off[kPhases * kSliceTile];
+    __shared__ int s_wave[kSliceTile / 64];
+    const int64_t tiles = ((int64_t)g.n + kSliceTile - 1) / kSliceTile;
+    for (int64_t T = blockIdx.x; T < tiles; T += gridDim.x) {
+        const int64_t first = T * kSliceTile;
+        const int64_t v = first + threadIdx.x;
+        const bool valid = v < g.n;
+        int s = 0, e = 0;
+        if (valid) {
+            s = g.ptr[v];
+            e = g.ptr[v + 1];
+        }
+        const bool big = valid && e - s > g.big_deg;
+        const bool mine = valid && !big;
+        int c[kPhases] = {};
+        if (mine) {
+            bool all_neg = true, any_neg = false;
+            for (int k0 = s; k0 < e; k0 += kSliceU) {
+                int2 ew[kSliceU];
 #pragma unroll
-                    for (int j = 0; j < kSliceU; j++) ew[j] = k0 + j < e ? g.edges[k0 + j] : make_int2(-1, 0);
+                for (int j = 0; j < kSliceU; j++) ew[j] = k0 + j < e ? g.edges[k0 + j] : make_int2(-1, 0);
 #pragma unroll
-                    for (int j = 0; j < kSliceU; j++) {
-                        if (ew[j].x < 0) break;
-                        const int sl = slice_of(ew[j].x, g.slice_len);
+                for (int j = 0; j < kSliceU; j++) {
+                    if (ew[j].x < 0) break;
+                    const int sl = slice_of(ew[j].x, g.slice_len);
 #pragma unroll
-                        for (int q = 0; q < kPhases; q++) c[q] += sl == q;
-                        all_neg &= ew[j].y < 0;
-                        any_neg |= ew[j].y < 0;
-                    }
+                    for (int q = 0; q < kPhases; q++) c[q] += sl == q;
+                    all_neg &= ew[j].y < 0;
+                    any_neg |= ew[j].y < 0;
                 }
-                // the initial activity of k_init, decided here in the same pass over the edges
-                g.stamp[v] = ((v < g.n0) ? all_neg : any_neg) ? 1 : 0;
             }
-#pragma unroll
-            for (int q = 0; q < kPhases; q++) cnt[(size_t)q * g.n + v] = c[q];
+            // lift(0)(v) > 0: owner 0 with every edge negative, owner 1 with a negative edge
+            g.stamp[v] = ((v < g.n0) ? all_neg : any_neg) ? 1 : 0;
         }
         const unsigned long long word = __ballot(big);
-        if ((threadIdx.x & 63) == 0 && v < g.n) bigmask[v >> 6] = word;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_slice_scatter(DevGame g, void *__restrict__ sedges) {
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
-        const int s = g.ptr[v], e = g.ptr[v + 1];
-        if (e - s > g.big_deg) continue;
-        int pos[kPhases];
+        if ((threadIdx.x & 63) == 0 && valid) bigmask[v >> 6] = word;
 #pragma unroll
-        for (int q = 0; q < kPhases; q++) pos[q] = g.sptr[(size_t)q * g.n + v];
-        for (int k0 = s; k0 < e; k0 += kSliceU) {
-            int2 ew[kSliceU];
+        for (int q = 0; q < kPhases; q++) s_off[q * kSliceTile + threadIdx.x] = c[q];
+        __syncthreads();
+        // exclusive scan in (slice, vertex) order; thread t takes items t*kPhases .. +kPhases-1
+        int loc[kPhases], run = 0;
 #pragma unroll
-            for (int j = 0; j < kSliceU; j++) ew[j] = k0 + j < e ? g.edges[k0 + j] : make_int2(-1, 0);
+        for (int k = 0; k < kPhases; k++) {
+            loc[k] = run;
+            run += s_off[threadIdx.x * kPhases + k];
+        }
+        int total;
+        const int ex = egs_block_exclusive_scan(run, s_wave, total);
 #pragma unroll
-            for (int j = 0; j < kSliceU; j++) {
-                if (ew[j].x < 0) break;
-                const int sl = slice_of(ew[j].x, g.slice_len);
+        for (int k = 0; k < kPhases; k++) s_off[threadIdx.x * kPhases + k] = ex + loc[k];
+        __syncthreads();
+        const int base = g.ptr[first];
+        int *out = sptr + T * kSliceStride;
 #pragma unroll
-                for (int q = 0; q < kPhases; q++) {
-                    if (sl != q) continue;
-                    if (g.sshift) ((uint32_t *)sedges)[pos[q]++] = ((uint32_t)ew[j].y << g.sshift) | (uint32_t)ew[j].x;
-                    else ((int2 *)sedges)[pos[q]++] = ew[j];
+        for (int q = 0; q < kPhases; q++) out[q * kSliceTile + threadIdx.x] = base + s_off[q * kSliceTile + threadIdx.x];
+        if (threadIdx.x == 0) out[kPhases * kSliceTile] = base + total;
+        if (mine) {
+            int pos[kPhases];
+#pragma unroll
+            for (int q = 0; q < kPhases; q++) pos[q] = base + s_off[q * kSliceTile + threadIdx.x];
+            for (int k0 = s; k0 < e; k0 += kSliceU) {
+                int2 ew[kSliceU];
+#pragma unroll
+                for (int j = 0; j < kSliceU; j++) ew[j] = k0 + j < e ? g.edges[k0 + j] : make_int2(-1, 0);
+#pragma unroll
+                for (int j = 0; j < kSliceU; j++) {
+                    if (ew[j].x < 0) break;
+                    const int sl = slice_of(ew[j].x, g.slice_len);
+#pragma unroll
+                    for (int q = 0; q < kPhases; q++) {
+                        if (sl != q) continue;
+                        if (g.sshift) ((uint32_t *)sedges)[pos[q]++] = ((uint32_t)ew[j].y << g.sshift) | (uint32_t)ew[j].x;
+                        else ((int2 *)sedges)[pos[q]++] = ew[j];
+                    }
                 }
             }
         }
+        __syncthreads();    // s_off is rewritten for the next tile
     }
 }
 
@@ -792,15 +818,11 @@ hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int r, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_slices(const DevGame &g, int *tile_sum, hipStream_t st) {
+hipError_t launch_slices(const DevGame &g, hipStream_t st) {
     if (g.phases <= 1) return hipSuccess;
-    int *cnt = const_cast<int *>(g.sptr);
-    const int64_t total = (int64_t)g.phases * g.n;
-    hipLaunchKernelGGL(k_slice_count, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, st, g, cnt,
+    hipLaunchKernelGGL(k_slice_build, dim3(grid_for(g.n, kSliceTile, 4096)), dim3(kSliceTile), 0, st, g,
+                       const_cast<int *>(g.sptr), const_cast<void *>(g.sedges),
                        const_cast<unsigned long long *>(g.bigmask));
-    launch_scan_inplace(cnt, (int)total, tile_sum, st);
-    hipLaunchKernelGGL(k_slice_scatter, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, st, g,
-                       const_cast<void *>(g.sedges));
     return hipGetLastError();
 }
 

@@ -36,29 +36,6 @@ constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 
-__device__ __forceinline__ int block_exclusive_scan(int x, int *lds_warp, int &total) {
-    // wave-level inclusive scan with shuffles, then wave totals combined through LDS.
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
-    int inc = x;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int y = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += y;
-    }
-    if (lane == 63) lds_warp[wid] = inc;
-    __syncthreads();
-    int wave_off = 0;
-    total = 0;
-    for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
-        int s = lds_warp[i];
-        if (i < wid) wave_off += s;
-        total += s;
-    }
-    __syncthreads();
-    return wave_off + inc - x;
-}
-
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const int *__restrict__ cnt, int n,
                                                              int *__restrict__ tile_sum) {
     __shared__ int lds[SCAN_THREADS / 64];
@@ -70,7 +47,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const int *__restri
         if (i < n) s += cnt[i];
     }
     int total;
-    block_exclusive_scan(s, lds, total);
+    egs_block_exclusive_scan(s, lds, total);
     if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
 }
 
@@ -81,7 +58,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_sums(int *__restrict__ ti
         int i = b + threadIdx.x;
         int x = (i < tiles) ? tile_sum[i] : 0;
         int total;
-        int ex = block_exclusive_scan(x, lds, total);
+        int ex = egs_block_exclusive_scan(x, lds, total);
         if (i < tiles) tile_sum[i] = carry + ex;
         carry += total;
     }
@@ -110,7 +87,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int *__restri
         run += items[threadIdx.x * SCAN_ITEMS + k];
     }
     int total;
-    int off = block_exclusive_scan(run, lds, total) + tile_off[blockIdx.x];
+    int off = egs_block_exclusive_scan(run, lds, total) + tile_off[blockIdx.x];
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) items[threadIdx.x * SCAN_ITEMS + k] = off + mine[k];
     __syncthreads();
@@ -288,15 +265,6 @@ hipError_t launch_pack(const int *succ, const int *w, int2 *edges, int64_t m, hi
     if (m > 0)
         hipLaunchKernelGGL(k_pack_edges, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st, succ, w, edges, m);
     return hipGetLastError();
-}
-
-// Exclusive scan of cnt[0..n) into cnt itself, with cnt[n] = the total (k_scan_apply
-// reads each tile into LDS before writing it back).
-void launch_scan_inplace(int *cnt, int n, int *tile_sum, hipStream_t st) {
-    const int tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(SCAN_THREADS), 0, st, cnt, n, tile_sum);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_THREADS), 0, st, tile_sum, tiles);
-    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(SCAN_THREADS), 0, st, cnt, n, tile_sum, cnt);
 }
 
 hipError_t launch_transpose(const DevGame &g, int *tile_sum, int *fill, hipStream_t st) {
