@@ -448,8 +448,8 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     g.hflag = s->d_hflag;
     g.bigmask = s->d_bigmask;
     // one lane per vertex up to two edges per slice on average; eight 256-thread workgroups
-    // per CU (a sweep on the 2^22-vertex game: 268 us per round against 282 with two lanes
-    // and 16 per CU, 352 with 9 per CU)
+    // per CU (a sweep on the 2^22-vertex game: 269 us per round against 282 with two lanes
+    // and 16 per CU, 352 with 9 per CU; 253 us since a lane keeps two edges in flight)
     int pg = 1;
     while (pg < 8 && pg * egs::kPhases * 2 < avg) pg <<= 1;
     sh.phase_group = pg;

@@ -491,6 +491,8 @@ __device__ __forceinline__ void slice_edge(const DevGame &g, int k, int &succ, i
     }
 }
 
+constexpr int kPhaseU = 2;   // edges in flight per lane in k_phase (4: same rounds, slower solves)
+
 template <int G, bool PACK>
 __global__ __launch_bounds__(256) void k_phase(DevGame g, int r, int s) {
     __shared__ int sbuf[kLdsCap];
@@ -516,10 +518,17 @@ __global__ __launch_bounds__(256) void k_phase(DevGame g, int r, int s) {
         }
         const bool owner1 = v >= g.n0;
         int key = EGS_TOP;
-        for (int k = b + gl; k < e; k += G) {
-            int u, w;
-            slice_edge<PACK>(g, k, u, w);
-            key = min(key, egs_key(egs_ominus(g.val[u], w, g.mg), owner1));
+        // kPhaseU edges per lane at a time: their loads, then their gathers, are all in
+        // flight together (an edge past e re-reads edge k0, a valid index, and is masked)
+        for (int k0 = b + gl; k0 < e; k0 += G * kPhaseU) {
+            int u[kPhaseU], w[kPhaseU], x[kPhaseU];
+#pragma unroll
+            for (int j = 0; j < kPhaseU; j++) slice_edge<PACK>(g, k0 + j * G < e ? k0 + j * G : k0, u[j], w[j]);
+#pragma unroll
+            for (int j = 0; j < kPhaseU; j++) x[j] = k0 + j * G < e ? g.val[u[j]] : EGS_TOP;
+#pragma unroll
+            for (int j = 0; j < kPhaseU; j++)
+                if (k0 + j * G < e) key = min(key, egs_key(egs_ominus(x[j], w[j], g.mg), owner1));
         }
         key = egs_group_min<G>(key);
         bool lifted = false;
