@@ -110,6 +110,7 @@ constexpr int kCntSlots = 64;
 
 constexpr int kBigU = 4;   // edges in flight per lane in the workgroup-per-vertex kernels
 constexpr int kClosureU = 4;   // edges in flight per thread in k_closure (one thread per vertex)
+constexpr int kBfU = 2;        // edges in flight per lane in k_bf_pass
 
 // Closure check, member count and the first pass of the longest-path game, in one sweep
 // over the values.  A member that breaks closure (an owner-0 edge leaving D and not to TOP,
@@ -299,18 +300,33 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
             kb = g.ptr[v];
             ke = g.ptr[v + 1];
             if (ke - kb > g.big_deg) ke = kb;         // k_bf_big evaluates it: no need here
-            for (int k = kb + gl; k < ke && !need; k += G) need = bm_test(prev, g.n, g.edges[k].x);
+            // kBfU edges per lane at a time (an edge past ke re-tests edge k0, a valid index)
+            for (int k0 = kb + gl; k0 < ke && !need; k0 += G * kBfU) {
+                int u[kBfU];
+#pragma unroll
+                for (int j = 0; j < kBfU; j++) u[j] = g.edges[k0 + j * G < ke ? k0 + j * G : k0].x;
+#pragma unroll
+                for (int j = 0; j < kBfU; j++) need |= bm_test(prev, g.n, u[j]);
+            }
         }
         need = ((__ballot(need) >> gbase) & kGroupMask) != 0;
         long long best = owner1 ? LLONG_MAX : LLONG_MIN;
         bool out = false;                    // an edge leaves D and does not reach TOP
         if (member && need) {
-            for (int k = kb + gl; k < ke; k += G) {
-                const int2 ew = g.edges[k];
-                const long long hu = h[ew.x];
-                if (hu == kHOut) { out = true; continue; }
-                const long long c = hu == kHTop ? LLONG_MIN : scale * ew.y + 1 + hu;
-                best = owner1 ? min(best, c) : max(best, c);
+            for (int k0 = kb + gl; k0 < ke; k0 += G * kBfU) {   // loads, then gathers, in flight together
+                int2 ew[kBfU];
+                long long hu[kBfU];
+#pragma unroll
+                for (int j = 0; j < kBfU; j++) ew[j] = g.edges[k0 + j * G < ke ? k0 + j * G : k0];
+#pragma unroll
+                for (int j = 0; j < kBfU; j++) hu[j] = k0 + j * G < ke ? h[ew[j].x] : kHTop;
+#pragma unroll
+                for (int j = 0; j < kBfU; j++) {
+                    if (k0 + j * G >= ke) break;
+                    if (hu[j] == kHOut) { out = true; continue; }
+                    const long long c = hu[j] == kHTop ? LLONG_MIN : scale * ew[j].y + 1 + hu[j];
+                    best = owner1 ? min(best, c) : max(best, c);
+                }
             }
         }
         out = ((__ballot(out) >> gbase) & kGroupMask) != 0;
