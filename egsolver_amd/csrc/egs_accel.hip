@@ -204,6 +204,112 @@ __global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, lon
 }
 
 
+// k_closure over the phased layout (games whose dense rounds run in successor-range
+// phases): phase s folds each member's edges into slice s, so its value gathers come from
+// one slice that stays in L2.  The fold is the one of k_closure (an OR, an AND and a max or
+// min, so the order of the edges does not matter); between phases a member's partial best
+// is kept in h[v] and its flags in acc[v] (bit 0: an edge reaches D or TOP, bit 1: an edge
+// leaves D and does not reach TOP).  Membership depends only on values and rise stamps,
+// which no phase changes.  The last phase completes the fold and then does what k_closure
+// does after its edge loop, for every vertex.
+constexpr int kClosurePhaseU = 2;   // edges in flight per thread (about two per slice)
+
+template <bool PACK>
+__global__ __launch_bounds__(256) void k_closure_phase(DevGame g, int B, int last, long long bound, int s,
+                                                       long long *__restrict__ h, unsigned *__restrict__ bits,
+                                                       unsigned *__restrict__ clear, int *__restrict__ cnt,
+                                                       int *__restrict__ flags) {
+    const long long scale = (long long)g.n + 1;
+    const int lane = threadIdx.x & 63;
+    const bool final_phase = s == g.phases - 1;
+    int members = 0, changed = 0, removed = 0;
+    // the stride is a multiple of 64, so a wave always covers 64 aligned vertices (two words)
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
+        const int x = g.val[v];
+        const bool member = in_d_x(g, (int)v, x, B, last);
+        const bool small = g.ptr[v + 1] - g.ptr[v] <= g.big_deg;
+        const bool owner1 = v >= g.n0;
+        long long best = owner1 ? LLONG_MAX : LLONG_MIN;
+        int fl = 0;
+        if (member && small) {
+            if (s > 0) {
+                best = h[v];
+                fl = g.acc[v];
+            }
+            const size_t i = slice_seg(v, s);
+            const int b = g.sptr[i], e = g.sptr[i + 1];
+            for (int k0 = b; k0 < e; k0 += kClosurePhaseU) {
+                int u[kClosurePhaseU], w[kClosurePhaseU], xu[kClosurePhaseU];
+#pragma unroll
+                for (int j = 0; j < kClosurePhaseU; j++) slice_edge<PACK>(g, k0 + j < e ? k0 + j : k0, u[j], w[j]);
+#pragma unroll
+                for (int j = 0; j < kClosurePhaseU; j++) xu[j] = k0 + j < e ? g.val[u[j]] : EGS_TOP;
+#pragma unroll
+                for (int j = 0; j < kClosurePhaseU; j++) {
+                    if (k0 + j >= e) break;
+                    if (xu[j] == EGS_TOP) {              // ends the path: -infinity for the game
+                        fl |= 1;
+                        if (owner1) best = LLONG_MIN;
+                    } else if (in_d_x(g, u[j], xu[j], B, last)) {
+                        fl |= 1;
+                        const long long c = scale * w[j] + 1 + scale * ((long long)g.mg - xu[j]);
+                        best = owner1 ? min(best, c) : max(best, c);
+                    } else {
+                        fl |= 2;
+                    }
+                }
+            }
+            if (!final_phase) {
+                h[v] = best;
+                g.acc[v] = fl;
+            }
+        }
+        if (!final_phase) continue;
+        long long hv;
+        bool up = false;
+        if (x == EGS_TOP) {
+            hv = kHTop;
+        } else if (!member) {
+            hv = kHOut;
+        } else if (!small) {
+            members++;                               // decided by k_closure_big
+            hv = scale * ((long long)g.mg - x);
+        } else {
+            members++;
+            const long long h0 = scale * ((long long)g.mg - x);
+            if (owner1 ? !(fl & 1) : (fl & 2)) {
+                hv = kHOut;
+                up = true;
+                removed++;
+            } else if (best > h0) {
+                hv = best;
+                up = true;
+                if (best > bound) atomicOr(&flags[1], 1);
+            } else {
+                hv = h0;
+            }
+        }
+        h[v] = hv;
+        changed += up;
+        const unsigned long long bal = __ballot(up);
+        if (lane == 0 || lane == 32) {
+            const int64_t word = v >> 5;
+            const unsigned bw = (unsigned)(bal >> lane);
+            bits[word] = bw;
+            if (bw) atomicOr(&bits[bm_words(g.n) + (word >> 5)], 1u << (word & 31));
+            clear[word] = 0u;
+        }
+    }
+    if (!final_phase) return;
+    for (int64_t i = bm_words(g.n) + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < bm_total(g.n);
+         i += (int64_t)gridDim.x * blockDim.x)
+        clear[i] = 0u;
+    block_add(members, &flags[3]);
+    block_add(removed, &flags[0]);
+    block_add(changed, &cnt[0]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[1] = 0;
+}
+
 // k_closure for the vertices above big_deg (launched after it), one workgroup each: the same
 // closure test and first pass; changes set their bit with an atomic OR, since k_closure has
 // stored every word already.
@@ -651,8 +757,19 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
         if (hipMemsetAsync(flags, 0, sizeof(int) * 4, st) != hipSuccess) return -1;
         if (hipMemsetAsync(cb.bfcnt, 0, sizeof(int) * kCntSlots, st) != hipSuccess) return -1;
         if (hipMemsetAsync(bm[0] + words, 0, sizeof(unsigned) * (bm_len - words), st) != hipSuccess) return -1;
-        hipLaunchKernelGGL(k_closure, dim3(grid), dim3(256), 0, st, g, B, last, bound, cb.h, bm[0], bm[1], cb.bfcnt,
-                           flags);
+        if (g.phases > 1) {
+            for (int s = 0; s < g.phases; s++) {
+                if (g.sshift)
+                    hipLaunchKernelGGL(k_closure_phase<true>, dim3(grid), dim3(256), 0, st, g, B, last, bound, s, cb.h,
+                                       bm[0], bm[1], cb.bfcnt, flags);
+                else
+                    hipLaunchKernelGGL(k_closure_phase<false>, dim3(grid), dim3(256), 0, st, g, B, last, bound, s, cb.h,
+                                       bm[0], bm[1], cb.bfcnt, flags);
+            }
+        } else {
+            hipLaunchKernelGGL(k_closure, dim3(grid), dim3(256), 0, st, g, B, last, bound, cb.h, bm[0], bm[1], cb.bfcnt,
+                               flags);
+        }
         if (has_big)
             hipLaunchKernelGGL(k_closure_big, dim3(kBigGrid), dim3(256), 0, st, g, B, last, bound, cb.h, bm[0], cb.bfcnt,
                                flags);

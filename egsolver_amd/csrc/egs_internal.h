@@ -57,7 +57,7 @@ struct DevGame {
     const void *sedges;   // uint32 (w << sshift) | succ when sshift > 0, else int2 {succ, w}
     int sshift;
     const unsigned long long *bigmask;   // bit v: vertex v is above big_deg (lifted by the wave/block kernels)
-    int *acc;             // n partial keys carried between phases
+    int *acc;             // n partial keys carried between phases (and the closure's flags)
     int *hacc, *hcnt;     // n each: partial keys and finished chunks of the vertices above
                           // huge_deg (TOP and 0 between rounds)
     long long *hbest;     // n: partial longest-path candidates of the vertices above huge_deg
@@ -69,6 +69,25 @@ constexpr int kPhases = 4;                 // successor slices of a phased dense
 constexpr int kPhaseMinN = 1 << 20;        // below this the value array fits the L2 anyway
 constexpr int kSliceTile = 256;            // vertices per tile of the slice layout
 constexpr int kSliceStride = kPhases * kSliceTile + 1;
+
+// Index in sptr of the segment (slice s, vertex v); the segment ends at the next index.
+__device__ __forceinline__ size_t slice_seg(int64_t v, int s) {
+    return (size_t)(v / kSliceTile) * kSliceStride + (size_t)s * kSliceTile + (size_t)(v % kSliceTile);
+}
+
+// Edge k of the slice layout: successor and weight.
+template <bool PACK>
+__device__ __forceinline__ void slice_edge(const DevGame &g, int k, int &succ, int &w) {
+    if (PACK) {
+        const uint32_t e = ((const uint32_t *)g.sedges)[k];
+        succ = (int)(e & ((1u << g.sshift) - 1u));
+        w = ((int)e) >> g.sshift;
+    } else {
+        const int2 e = ((const int2 *)g.sedges)[k];
+        succ = e.x;
+        w = e.y;
+    }
+}
 
 // Launch shape of one round.
 struct LaunchShape {

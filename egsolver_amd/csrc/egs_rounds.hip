@@ -478,19 +478,6 @@ __global__ __launch_bounds__(256) void k_round(DevGame g, int r) {
 // every phase (the round is Jacobi within itself), the last writes the risen ones, stamps
 // them and stages them for the next round like body_dense.  Vertices above big_deg have
 // no edges in the slices: the static wave/block kernels lift them after the phases.
-template <bool PACK>
-__device__ __forceinline__ void slice_edge(const DevGame &g, int k, int &succ, int &w) {
-    if (PACK) {
-        const uint32_t e = ((const uint32_t *)g.sedges)[k];
-        succ = (int)(e & ((1u << g.sshift) - 1u));
-        w = ((int)e) >> g.sshift;
-    } else {
-        const int2 e = ((const int2 *)g.sedges)[k];
-        succ = e.x;
-        w = e.y;
-    }
-}
-
 constexpr int kPhaseU = 2;   // edges in flight per lane in k_phase (4: same rounds, slower solves)
 
 template <int G, bool PACK>
@@ -512,7 +499,7 @@ __global__ __launch_bounds__(256) void k_phase(DevGame g, int r, int s) {
         const bool valid = v < g.n;
         int b = 0, e = 0;
         if (valid) {
-            const size_t i = (size_t)(v / kSliceTile) * kSliceStride + (size_t)s * kSliceTile + v % kSliceTile;
+            const size_t i = slice_seg(v, s);
             b = g.sptr[i];
             e = g.sptr[i + 1];
         }
