@@ -555,7 +555,7 @@ __device__ __forceinline__ int slice_of(int u, int slice_len) {
 }
 
 // Edges read ahead per thread in the slice set-up kernel (one thread per vertex).
-constexpr int kSliceU = 4;
+constexpr int kSliceU = 8;
 
 // Slice layout of the low-degree vertices' edges in one pass, one tile of kSliceTile
 // vertices per workgroup iteration and no global scan: each thread counts its vertex's
