@@ -269,7 +269,9 @@ def test_phased_equals_single_sweep_full_size():
 
 def test_phased_unpacked_edges_equal_single_sweep():
     """Weights too wide to share a 32-bit word with a 17-bit successor id keep the slice
-    edges as int2; phased and single-sweep rounds end on the same fixpoint."""
+    edges as int2; phased and single-sweep rounds end on the same fixpoint.  Proofs run in
+    both (every 4 rounds), so the phased closure check over int2 slice edges is compared
+    with the one-sweep closure check."""
     import egsolver_amd as E
     g = E.generate(E.GenParams(n=1 << 17, seed=91, deg_kind=1, deg_lo=2, deg_hi=6, wmax=20000))
     _say(f"phased unpacked n {g.n} m {g.m} mg {g.mg}")
@@ -277,9 +279,10 @@ def test_phased_unpacked_edges_equal_single_sweep():
         s.load(g)
         out = []
         for sl in (1, 4):
-            s.prepare(E.options(slices=sl))
+            s.prepare(E.options(slices=sl, certify_every=4))
             conv, st = s.iterate()
             assert conv and s.violations() == 0
+            assert st.cert_attempts > 0 and st.cert_passes > 0, (sl, st.cert_attempts, st.cert_passes)
             out.append(s.values())
         assert np.array_equal(out[0], out[1])
 
