@@ -815,4 +815,100 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
     return 0;
 }
 
+// ---------------------------------------------------------------------------------------
+// Certificate of the whole TOP set of the current values
+// ---------------------------------------------------------------------------------------
+//
+// The proofs above mark sets of vertices TOP during a run; ordinary lifting marks others.
+// After a run, the longest-path game of step 3 is run once more with D = T, the set of all
+// TOP vertices, from h = 0, with no TOP sentinel (every TOP vertex is a member, so every
+// edge into T is constrained and none ends a path).  A fixpoint with nothing removed gives
+// potentials under which every owner-0 edge inside T and some edge into T of every owner-1
+// vertex of T satisfies h(v) >= (n+1) w + 1 + h(u), and no owner-0 vertex of T has an edge
+// leaving T: a strategy of player 1 that keeps every play from T inside T with every cycle
+// of w-weight <= -1, so T is won by player 1.  A host-side check of exactly these
+// inequalities proves T is contained in the least fixpoint's TOP set without trusting any of
+// this code; a fixpoint check of the values gives the converse.
+
+// Seeds pass 0: h = 0 on T and kHOut elsewhere, pass 1 evaluates every member (all bits of
+// `prev`), the bitmap pass 1 writes starts empty, cnt[0] = members (so pass 1 runs) and
+// flags[3] counts the members.
+__global__ __launch_bounds__(256) void k_cert_all_seed(DevGame g, long long *__restrict__ h,
+                                                       unsigned *__restrict__ prev, unsigned *__restrict__ cur,
+                                                       int *__restrict__ cnt, int *__restrict__ flags) {
+    int members = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
+        const bool top = g.val[v] == EGS_TOP;
+        h[v] = top ? 0 : kHOut;
+        members += top;
+    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < bm_total(g.n);
+         i += (int64_t)gridDim.x * blockDim.x) {
+        prev[i] = ~0u;
+        cur[i] = 0u;
+    }
+    block_add(members, &flags[3]);
+    block_add(members, &cnt[0]);
+}
+
+// Potentials for export: -1 outside the certified set.
+__global__ __launch_bounds__(256) void k_cert_export(int n, long long *__restrict__ h) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+        if (h[v] < 0) h[v] = -1;
+}
+
+int certify_all_top(const DevGame &g, const CertBuffers &cb, int wmax, int max_passes, bool has_big,
+                    hipStream_t st, CertResult &res) {
+    constexpr int kBigGrid = 512;
+    res = CertResult{};
+    const long long scale = (long long)g.n + 1;
+    // at a fixpoint from h = 0, h <= (n - 1) max w' along a path; twice that is divergence
+    const double wp = (double)scale * std::max(wmax, 0) + 1;
+    if (2.0 * g.n * wp > 4.0e18) return -2;
+    const long long bound = 2LL * g.n * ((long long)scale * std::max(wmax, 0) + 1);
+    const int grid = grid_for(g.n, 256, 2048);
+    int *flags = cb.hist + 2 * kBins;
+    int *fl = cb.pinned + 2 * kBins;
+    int *cnt = fl + 4;
+    const int64_t words = ((int64_t)g.n + 31) >> 5, bm_len = words + (words + 31) / 32;
+    unsigned *bm[3] = {cb.bits, cb.bits + bm_len, cb.bits + 2 * bm_len};
+    int G = 1;
+    while (G < 16 && 2 * G < (double)g.m / g.n) G <<= 1;
+    const int bf_grid = grid_for((int64_t)g.n * G, 256, 4096);
+    if (hipMemsetAsync(flags, 0, sizeof(int) * 4, st) != hipSuccess) return -1;
+    if (hipMemsetAsync(cb.bfcnt, 0, sizeof(int) * kCntSlots, st) != hipSuccess) return -1;
+    hipLaunchKernelGGL(k_cert_all_seed, dim3(grid), dim3(256), 0, st, g, cb.h, bm[0], bm[1], cb.bfcnt, flags);
+    int queued = 1, chunk = 8;
+    bool converged = false, diverged = false;
+    while (!converged && !diverged && queued <= max_passes) {
+        const int upto = std::min(max_passes + 1, queued + chunk);
+        for (int p = queued; p < upto; p++) {
+            launch_bf_pass(G, bf_grid, st, g, cb.h, bound, bm[(p - 1) % 3], bm[p % 3], bm[(p + 1) % 3], p, cb.bfcnt,
+                           flags);
+            if (has_big) {
+                hipLaunchKernelGGL(k_bf_big, dim3(kBigGrid), dim3(256), 0, st, g, cb.h, bound, bm[(p - 1) % 3],
+                                   bm[p % 3], p, cb.bfcnt, flags);
+                hipLaunchKernelGGL(k_bf_huge, dim3(2 * kBigGrid), dim3(256), 0, st, g, cb.h, bound, bm[(p - 1) % 3],
+                                   bm[p % 3], p, cb.bfcnt, flags);
+            }
+        }
+        if (hipMemcpyAsync(fl, flags, sizeof(int) * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+        if (hipMemcpyAsync(cnt, cb.bfcnt, sizeof(int) * kCntSlots, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+        if (hipStreamSynchronize(st) != hipSuccess) return -1;
+        if (fl[1]) diverged = true;
+        for (int p = queued; p < upto && !diverged; p++) {
+            res.passes = p;
+            if (cnt[p % kCntSlots] == 0) { converged = true; break; }
+        }
+        queued = upto;
+        chunk = std::min(chunk * 2, 256);
+    }
+    res.marked = fl[3];                     // TOP vertices
+    res.closure_failed = fl[0];             // TOP vertices that left the certified set
+    res.cap_hit = !converged && !diverged;
+    hipLaunchKernelGGL(k_cert_export, dim3(grid), dim3(256), 0, st, g.n, cb.h);
+    if (hipGetLastError() != hipSuccess) return -1;
+    return converged && !diverged && fl[0] == 0 ? 1 : 0;
+}
+
 }  // namespace egs

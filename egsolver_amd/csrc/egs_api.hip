@@ -658,6 +658,29 @@ int egs_solver_count_violations(egs_solver *s, int64_t *violations) {
     return EGS_OK;
 }
 
+int egs_solver_set_values(egs_solver *s, const int32_t *host_values) {
+    if (!s || !s->prepared || !host_values) return fail(EGS_E_STATE, "egs_solver_set_values: call egs_solver_prepare first");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemcpyAsync(s->d_val, host_values, sizeof(int) * (size_t)s->g.n, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return EGS_OK;
+}
+
+int egs_solver_top_certificate(egs_solver *s, int32_t max_passes, int64_t *host_h, int32_t *passes) {
+    if (!s || !s->prepared || !host_h) return fail(EGS_E_STATE, "egs_solver_top_certificate: call egs_solver_prepare first");
+    HIP_TRY(hipSetDevice(s->device));
+    egs::CertBuffers cb{s->d_hist, s->d_h, s->d_bfcnt, s->d_bits, s->h_probe + kPinCert};
+    egs::CertResult cr;
+    const int got = egs::certify_all_top(s->g, cb, s->wmax, max_passes > 0 ? max_passes : (1 << 16),
+                                         s->sh.use_wave != 0, s->stream, cr);
+    if (got == -1) return fail(EGS_E_HIP, "certificate: %s", hipGetErrorString(hipGetLastError()));
+    if (got == -2) return fail(EGS_E_ARG, "certificate: potentials of this game could overflow 64 bits");
+    HIP_TRY(hipMemcpyAsync(host_h, s->d_h, sizeof(int64_t) * (size_t)s->g.n, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (passes) *passes = cr.passes;
+    return got;
+}
+
 int egs_solve(int device, int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *ptr,
               const int32_t *succ, const int32_t *w, int32_t *values, const egs_options *opt,
               egs_stats *st) {

@@ -136,5 +136,11 @@ int gap_candidates(const int *win, const int *flo, int wmax, int mg, int64_t *ou
 hipError_t certify_prefetch(const DevGame &g, const CertBuffers &cb, int wmax, hipStream_t st);
 int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int cap, int max_cands, bool has_big,
                 bool hist_ready, hipStream_t st, CertResult &res);
+// Potentials certifying that every TOP vertex of the current values is won by player 1
+// (cb.h: >= 0 on the certified set, -1 elsewhere).  Returns 1 when the whole TOP set is
+// certified, 0 when not (closure failed, divergence or the pass cap; see res), -1 on a HIP
+// error, -2 when the potentials could overflow 64 bits.
+int certify_all_top(const DevGame &g, const CertBuffers &cb, int wmax, int max_passes, bool has_big,
+                    hipStream_t st, CertResult &res);
 
 }  // namespace egs

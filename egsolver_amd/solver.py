@@ -96,6 +96,23 @@ class Solver:
         N.check(N.egs().egs_solver_count_violations(self.h, ctypes.byref(v)), "egs_solver_count_violations")
         return int(v.value)
 
+    def set_values(self, values):
+        """Replace the prepared solver's values (for example to certify another path's result)."""
+        v = np.ascontiguousarray(values, np.int32)
+        if v.shape != (self.n,):
+            raise ValueError(f"need {self.n} values, got {v.shape}")
+        N.check(N.egs().egs_solver_set_values(self.h, v.ctypes.data), "egs_solver_set_values")
+
+    def top_certificate(self, max_passes=0):
+        """Potentials certifying that every TOP vertex of the current values is won by player 1
+        (int64[n], >= 0 exactly on the TOP set, -1 elsewhere).  Returns (certified, h, passes);
+        check them on the host with egsolver_amd.verify.check_top_certificate."""
+        h = np.empty(self.n, np.int64)
+        passes = ctypes.c_int32(0)
+        rc = N.check(N.egs().egs_solver_top_certificate(self.h, int(max_passes), h.ctypes.data, ctypes.byref(passes)),
+                     "egs_solver_top_certificate")
+        return bool(rc), h, int(passes.value)
+
     def stats(self):
         st = N.EgsStats()
         N.check(N.egs().egs_solver_stats(self.h, ctypes.byref(st)), "egs_solver_stats")

@@ -160,6 +160,23 @@ int egs_solve(int device, int32_t n, int32_t n0, int64_t m, int32_t mg, const in
  * Zero, for values reached by monotone lifting from 0, certifies the least fixpoint. */
 int egs_solver_count_violations(egs_solver *s, int64_t *violations);
 
+/* No reference counterpart.  Replace the prepared solver's values with host_values[n]
+ * (for example the result of the batched or serving path, to certify it below). */
+int egs_solver_set_values(egs_solver *s, const int32_t *host_values);
+
+/* No reference counterpart.  Certificate that every TOP vertex of the current values is
+ * won by player 1: potentials host_h[n] (int64) with host_h[v] >= 0 exactly on
+ * T = {v : value == TOP} and -1 elsewhere, such that, with w' = (n+1) w + 1,
+ *   every owner-0 vertex v of T has all successors u in T and h(v) >= w'(v,u) + h(u),
+ *   every owner-1 vertex v of T has some successor u in T with h(v) >= w'(v,u) + h(u).
+ * Such potentials exist exactly when T lies in player 1's winning region (the least
+ * fixpoint's TOP set): the chosen owner-1 edges keep every play in T, and every cycle
+ * under them has w-weight <= -1.  egs_check_top_certificate (egs_host.h) verifies them
+ * on the host in O(m).  Computed on the device by longest-path passes from h = 0, at most
+ * max_passes (<= 0: 65,536).  Returns 1 when T is certified, 0 when not (host_h then holds
+ * the partial potentials), or an error code; *passes (may be NULL) gets the passes run. */
+int egs_solver_top_certificate(egs_solver *s, int32_t max_passes, int64_t *host_h, int32_t *passes);
+
 /* ---- many games in one launch ---------------------------------------------------- */
 
 /* No reference counterpart (the reference solves one game per process).  Games are

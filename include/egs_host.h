@@ -130,6 +130,31 @@ int egs_csr_save(const char *path, int32_t n, int32_t n0, int64_t m, int32_t mg,
 int egs_csr_load(const char *path, egs_csr *out, char *errbuf, int errlen);
 void egs_csr_free(egs_csr *g);
 
+/* ---- result verification (no reference counterpart) ---------------------------------- */
+
+/* Vertices with lift(values)(v) != values(v), for the lifting of src/cpu_solver.c:300-305
+ * (owner 0 the min, owner 1 the max over out-edges of OMINUS, src/common.h:270-272), plus
+ * vertices without an out-edge.  *first_bad (may be NULL) gets the first such vertex or -1.
+ * Zero means values is a fixpoint, hence at or above the least fixpoint. */
+int64_t egs_check_fixpoint(int32_t n, int32_t n0, int32_t mg, const int32_t *ptr, const int32_t *succ,
+                           const int32_t *w, const int32_t *values, int64_t *first_bad);
+
+/* Checks a certificate (egs_solver_top_certificate in egs.h) that T = {v : values[v] ==
+ * TOP} is won by player 1: h[v] >= 0 exactly on T; with w' = (n+1) w + 1, every owner-0
+ * vertex of T has all successors in T and h(v) >= w'(v,u) + h(u) on each, and every owner-1
+ * vertex of T has a successor u in T with h(v) >= w'(v,u) + h(u) (exact 128-bit arithmetic).
+ * Returns EGS_CERT_OK or the first failure's code, with its vertex in *bad_vertex (-1 when
+ * none; may be NULL). */
+enum {
+    EGS_CERT_OK = 0,
+    EGS_CERT_MEMBERSHIP = 1, /* h >= 0 does not match value == TOP */
+    EGS_CERT_LEAVES = 2,     /* an owner-0 vertex of T has a successor outside T */
+    EGS_CERT_POTENTIAL = 3,  /* an owner-0 edge inside T breaks the inequality */
+    EGS_CERT_NO_EDGE = 4     /* an owner-1 vertex of T has no edge into T that meets it */
+};
+int egs_check_top_certificate(int32_t n, int32_t n0, const int32_t *ptr, const int32_t *succ, const int32_t *w,
+                              const int32_t *values, const int64_t *h, int64_t *bad_vertex);
+
 /* Write a game in .mpg format (read order, external ids 0..n-1); gz != 0 for gzip. */
 int egs_write_mpg(const char *path, int n, const uint8_t *owner, const int32_t *rptr,
                   const int32_t *rsucc, const int32_t *rw, int gz);

@@ -18,6 +18,14 @@
  *                                 and the compiled reference host_csr2csc.c.
  *   ora_max_loop                : closed forms incl. the LONG_MAX-1 saturation.
  *   ora_fixpoint_violations     : planted non-fixpoints must be rejected.
+ *   ora_lift_all                : one step of the brute-force Kleene iteration (oracle.py),
+ *                                 lift(lfp) == lfp, and lift(0) against one EG0 sweep.
+ *   ora_initial_active          : equals lift(0)(v) > 0 (the closed form of
+ *                                 src/cpu_solver.c:217-249) on games of mixed-sign weights
+ *                                 for both owners, and hand-built single-vertex cases.
+ *   ora_eg0_from                : from 0 it is EG0; seeded with TOP on the lfp's TOP set it
+ *                                 returns the lfp in fewer sweeps; seeded above the lfp it
+ *                                 keeps the planted value (the seed is used).
  */
 #include <limits.h>
 #include <stdint.h>
@@ -216,6 +224,38 @@ long ora_eg0(int n, int n0, int m, int mg, const int *ptr, const int *succ, cons
     return loop - 1;
 }
 
+/* EG0 from a start vector: the Jacobi sweeps of ora_eg0 (src/cpu_solver.c:109-185) with
+ * both buffers seeded from `start` instead of zeros (the reference always starts at 0).
+ * From any start f0 <= lfp the sweeps rise to the least fixpoint, since lifting is monotone
+ * and lift(lfp) = lfp; the tests seed TOP on a set already proved to be player 1's and 0
+ * elsewhere, which leaves only the finite values to compute.  Stops on a sweep with no
+ * change or after max_sweeps (< 0: none); `out` gets the last buffer written, the return
+ * value is the sweep count. */
+long ora_eg0_from(int n, int n0, int mg, const int *ptr, const int *succ, const int *w,
+                  const int *start, int *out, long max_sweeps, int threads) {
+    int *b1 = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+    int *b2 = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+    memcpy(b1, start, sizeof(int) * (size_t)n);
+    memcpy(b2, start, sizeof(int) * (size_t)n);
+    if (threads > 0) omp_set_num_threads(threads);
+    long sweeps = 0;
+    for (;;) {
+        if (max_sweeps >= 0 && sweeps >= max_sweeps) break;
+        int changed = 0;
+#pragma omp parallel for schedule(guided, 4) reduction(| : changed)
+        for (int v = 0; v < n; v++) {
+            int t = ora_lift_vertex(v, n0, mg, ptr, succ, w, b1);
+            if (b2[v] < t) { b2[v] = t; changed = 1; }
+        }
+        int *tmp = b1; b1 = b2; b2 = tmp;
+        sweeps++;
+        if (!changed) break;
+    }
+    memcpy(out, b1, sizeof(int) * (size_t)n);
+    free(b1); free(b2);
+    return sweeps;
+}
+
 /* Fixpoint certificate: returns the number of vertices with lift(f)(v) != f(v).  Zero
  * together with "f was reached by monotone lifting from 0" certifies f = lfp. */
 long ora_fixpoint_violations(int n, int n0, int mg, const int *ptr, const int *succ,
@@ -232,35 +272,4 @@ long ora_fixpoint_violations(int n, int n0, int mg, const int *ptr, const int *s
 void ora_lift_all(int n, int n0, int mg, const int *ptr, const int *succ, const int *w,
                   const int *val, int *out) {
     for (int v = 0; v < n; v++) out[v] = ora_lift_vertex(v, n0, mg, ptr, succ, w, val);
-}
-
-/* Owner partition of the read order: src/egsolver.c:146-214 with the default
- * comparator prima0poi1 (src/sorting_criteria.h:91-102) under glibc's merge-sort
- * qsort, which is stable: owner-0 vertices in read order, then owner-1 vertices in read
- * order.  mapping[internal] = read index, revmapping[read index] = internal. */
-int ora_partition(int n, const unsigned char *owner, int *mapping, int *revmapping) {
-    int k = 0;
-    for (int i = 0; i < n; i++) if (owner[i] == 0) mapping[k++] = i;
-    int n0 = k;
-    for (int i = 0; i < n; i++) if (owner[i] != 0) mapping[k++] = i;
-    for (int i = 0; i < n; i++) revmapping[mapping[i]] = i;
-    return n0;
-}
-
-/* Internal CSR: src/egsolver.c:253-281.  Row i of the internal CSR is read-order row
- * mapping[i]; successor names go external -> read index -> internal. */
-void ora_build_internal(int n, const int *rptr, const int *rsucc_ext, const int *rw,
-                        const int *int_of_ext, const int *mapping, const int *revmapping,
-                        int *ptr, int *succ, int *w) {
-    int e = 0;
-    for (int i = 0; i < n; i++) {
-        ptr[i] = e;
-        int r = mapping[i];
-        for (int j = rptr[r]; j < rptr[r + 1]; j++) {
-            succ[e] = revmapping[int_of_ext[rsucc_ext[j]]];
-            w[e] = rw[j];
-            e++;
-        }
-    }
-    ptr[n] = e;
 }

@@ -272,6 +272,9 @@ class Oracle:
         L.ora_fixpoint_violations.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p,
                                               _i32p, _i32p]
         L.ora_fixpoint_violations.restype = ctypes.c_long
+        L.ora_eg0_from.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _i32p, _i32p, _i32p,
+                                   ctypes.c_long, ctypes.c_int]
+        L.ora_eg0_from.restype = ctypes.c_long
         L.ora_lift_all.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _i32p,
                                    _i32p, _i32p]
 
@@ -312,6 +315,14 @@ class Oracle:
         sweeps = self.L.ora_eg0(g.n, g.n0, g.m, g.mg if mg is None else mg, g.ptr, g.succ, g.w,
                                 val, max_loop, threads, None)
         return val, sweeps
+
+    def eg0_from(self, g: Game, start, max_sweeps=-1, threads=0):
+        """EG0 Jacobi sweeps from `start` (TOP on a set already proved player 1's, 0 elsewhere,
+        gives the least fixpoint).  Returns (values, sweeps)."""
+        start = np.ascontiguousarray(start, np.int32)
+        out = np.zeros(g.n, np.int32)
+        sweeps = self.L.ora_eg0_from(g.n, g.n0, g.mg, g.ptr, g.succ, g.w, start, out, max_sweeps, threads)
+        return out, sweeps
 
     def fixpoint_violations(self, g: Game, val, mg=None):
         val = np.ascontiguousarray(val, np.int32)

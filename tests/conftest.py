@@ -44,3 +44,27 @@ def lfp(oracle, og, threads=4):
         v1, _ = oracle.eg(og)
         assert np.array_equal(v0, v1), "oracle EG and EG0 disagree"
     return v0
+
+
+def exact_by_certificate(oracle, g, vals, solver):
+    """Full-size parity without trusting the device's TOP proofs.
+
+    1. The host fixpoint check (lift(f) == f): f lies at or above the least fixpoint, so the
+       least fixpoint's TOP set is inside T = {f == TOP}.
+    2. A certificate for T computed on the device (solver holds g, prepared) and checked on
+       the host: T is won by player 1, so T is inside the least fixpoint's TOP set.
+    3. With T pinned, the oracle's EG0 from (TOP on T, 0 elsewhere) rises to the least
+       fixpoint (a start below it), and f must equal it bit for bit.
+    Returns the certificate report."""
+    import numpy as np
+    from egsolver_amd import verify
+    rep = verify.certify(solver, g, vals)
+    assert rep["fixpoint_violations"] == 0, rep
+    assert rep["certificate_found"], rep
+    assert rep["certificate_code"] == 0, (verify.CERT_CODES[rep["certificate_code"]], rep)
+    og = to_oracle_game(g)
+    start = np.where(vals == 2147483647, 2147483647, 0).astype(np.int32)
+    ref, sweeps = oracle.eg0_from(og, start)
+    assert np.array_equal(vals, ref), f"{int((vals != ref).sum())} values differ from the oracle's seeded EG0"
+    rep["oracle_sweeps"] = int(sweeps)
+    return rep

@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, lfp, load_golden_sol, to_oracle_game
+from conftest import GOLDEN, exact_by_certificate, lfp, load_golden_sol, to_oracle_game
 
 pytestmark = pytest.mark.gpu
 
@@ -249,9 +249,12 @@ def test_winning_set_bitmap(oracle):
             assert np.array_equal(s.winning(), ref != TOP)
 
 
-def test_phased_equals_single_sweep_full_size():
-    """Config 2 at full size: phased and single-sweep dense rounds end on the same values
-    (no oracle finishes this size; the device fixpoint check covers it)."""
+def test_config2_full_size_exact(oracle):
+    """Config 2 at full size (n = 2^22, m = 2^25), phased and single-sweep dense rounds.
+    No oracle reaches this fixpoint from 0 (about 10^6 sweeps), so the result is pinned
+    without trusting the device's TOP proofs: host fixpoint check, a device certificate of
+    the whole TOP set checked on the host, then the oracle's EG0 from the certified TOP set
+    must give the same values bit for bit."""
     import egsolver_amd as E
     g = E.config_game(2)
     _say(f"config 2 full size, phased vs single sweep, n {g.n} m {g.m}")
@@ -262,9 +265,13 @@ def test_phased_equals_single_sweep_full_size():
             s.prepare(E.options(slices=sl))
             conv, st = s.iterate()
             assert conv
+            assert st.certificates >= 1
             assert s.violations() == 0
             out.append(s.values())
         assert np.array_equal(out[0], out[1])
+        rep = exact_by_certificate(oracle, g, out[1], s)
+        _say(f"config 2 certified: {rep}")
+        assert rep["top_vertices"] > g.n // 3
 
 
 def test_phased_unpacked_edges_equal_single_sweep():
@@ -287,9 +294,11 @@ def test_phased_unpacked_edges_equal_single_sweep():
         assert np.array_equal(out[0], out[1])
 
 
-def test_config3_full_size_fixpoint():
+def test_config3_full_size_exact(oracle):
     """The skewed-degree game at full size (n = 2^20, power-law out-degrees up to 10^5):
-    fixpoint certificate, and phased vs single-sweep dense rounds agree."""
+    phased and single-sweep dense rounds agree, and the result is pinned as for config 2
+    (host fixpoint check, certificate of the TOP set checked on the host, oracle EG0 from the
+    certified TOP set)."""
     import egsolver_amd as E
     g = E.config_game(3)
     _say(f"config 3 full size n {g.n} m {g.m}")
@@ -300,5 +309,38 @@ def test_config3_full_size_fixpoint():
             s.prepare(E.options(slices=sl))
             conv, st = s.iterate()
             assert conv and s.violations() == 0
+            assert st.certificates >= 1
             out.append(s.values())
         assert np.array_equal(out[0], out[1])
+        rep = exact_by_certificate(oracle, g, out[0], s)
+        _say(f"config 3 certified: {rep}")
+
+
+def test_top_certificate_small_games(oracle):
+    """The certificate of the TOP set on games the oracle solves from 0: found and accepted
+    by the host check on the exact values; on values with a finite vertex raised to TOP it
+    is not found and the host check rejects whatever potentials come back."""
+    import egsolver_amd as E
+    from egsolver_amd import verify
+    rng = np.random.default_rng(7)
+    for i, (n, lo, hi, wm) in enumerate([(2000, 4, 4, 10), (3000, 1, 6, 20), (500, 2, 3, 5), (5000, 1, 12, 100),
+                                         (20000, 2, 16, 300)]):
+        g = E.generate(E.GenParams(n=n, seed=100 + i, deg_kind=1 if lo != hi else 0, deg_lo=lo, deg_hi=hi, wmax=wm))
+        _say(f"certificate small game {i} n {g.n} m {g.m}")
+        ref = lfp(oracle, to_oracle_game(g))
+        with E.Solver(g.n, g.m) as s:
+            s.load(g)
+            s.prepare()
+            conv, st = s.iterate()
+            vals = s.values()
+            assert conv and np.array_equal(vals, ref)
+            rep = verify.certify(s, g, vals)
+            assert rep["ok"], rep
+            finite = np.nonzero(vals != TOP)[0]
+            if len(finite):
+                bad = vals.copy()
+                bad[rng.choice(finite)] = TOP
+                s.set_values(bad)
+                found, h, _ = s.top_certificate(max_passes=4096)
+                code, _ = verify.check_top_certificate(g, bad, h)
+                assert not found and code != 0

@@ -284,3 +284,80 @@ def test_eg_and_eg0_agree_on_larger_random(oracle):
         b, _ = oracle.eg0(g, threads=4)
         assert np.array_equal(a, b)
         assert oracle.fixpoint_violations(g, a) == 0
+
+
+# ---- one lift, initial frontier, seeded EG0 -----------------------------------------------
+
+def _kleene_step(g, f):
+    """One step of the brute-force Kleene iteration, straight from the definition."""
+    out = []
+    for v in range(g.n):
+        xs = []
+        for e in range(g.ptr[v], g.ptr[v + 1]):
+            a, w = int(f[g.succ[e]]), int(g.w[e])
+            xs.append(max(0, a - w) if (a < TOP and a - w <= g.mg) else TOP)
+        out.append(min(xs) if v < g.n0 else max(xs))
+    return np.array(out, np.int32)
+
+
+def test_lift_all_pinned(oracle):
+    """ora_lift_all: equal to a brute-force Kleene step on random values, the identity on the
+    least fixpoint, and lift(0) equal to one EG0 sweep from 0 (cpu_solver.c:109-185)."""
+    rng = np.random.default_rng(21)
+    for trial in range(60):
+        g = _rand_game(rng, int(rng.integers(2, 40)), 5, 6)
+        f = rng.integers(0, g.mg + 2, size=g.n).astype(np.int32)
+        f[rng.random(g.n) < 0.2] = TOP
+        assert np.array_equal(oracle.lift_all(g, f), _kleene_step(g, f))
+        lfp = kleene(g)
+        assert np.array_equal(oracle.lift_all(g, lfp), lfp)
+        one, _ = oracle.eg0(g, max_loop=2, threads=1)
+        z = np.zeros(g.n, np.int32)
+        assert np.array_equal(oracle.lift_all(g, oracle.lift_all(g, z)), one)
+
+
+def test_initial_active_mixed_signs(oracle):
+    """ora_initial_active (cpu_solver.c:217-249) equals lift(0)(v) > 0 on games whose vertices
+    have mixed-sign weights, for both owners; swapping the owner rules fails it."""
+    rng = np.random.default_rng(22)
+    seen = set()
+    for trial in range(80):
+        g = _rand_game(rng, int(rng.integers(2, 60)), 6, 4)
+        act = oracle.initial_active(g).astype(bool)
+        want = _kleene_step(g, np.zeros(g.n, np.int32)) > 0
+        assert np.array_equal(act, want)
+        for v in range(g.n):
+            ws = g.w[g.ptr[v]:g.ptr[v + 1]]
+            seen.add((v < g.n0, bool((ws < 0).any()), bool((ws >= 0).any()), bool(act[v])))
+    # every owner x sign pattern occurred: (owner0, has neg, has non-neg, active)
+    assert {(True, True, True, False), (False, True, True, True), (True, True, False, True),
+            (False, False, True, False)} <= seen
+    # hand-built: owner 0 with mixed weights is idle, owner 1 with mixed weights is active
+    g = game_from_arrays(2, 1, np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32),
+                         np.array([-1, 2, -1, 2], np.int32))
+    assert oracle.initial_active(g).tolist() == [0, 1]
+
+
+def test_eg0_from_seeded(oracle):
+    """ora_eg0_from: from 0 it is EG0; from TOP on the least fixpoint's TOP set it returns
+    the least fixpoint in no more sweeps; a planted value above the least fixpoint survives
+    (the seed is used, not zeroed)."""
+    rng = np.random.default_rng(23)
+    fewer = 0
+    for trial in range(60):
+        g = _rand_game(rng, int(rng.integers(2, 80)), 4, 8)
+        lfp, sweeps = oracle.eg0(g, threads=2)
+        a, s0 = oracle.eg0_from(g, np.zeros(g.n, np.int32), threads=2)
+        assert np.array_equal(a, lfp)
+        seed = np.where(lfp == TOP, TOP, 0).astype(np.int32)
+        b, s1 = oracle.eg0_from(g, seed, threads=2)
+        assert np.array_equal(b, lfp) and s1 <= s0
+        fewer += s1 < s0
+        fin = np.nonzero(lfp != TOP)[0]
+        if len(fin):
+            v = int(fin[0])
+            up = lfp.copy()
+            up[v] = TOP
+            c, _ = oracle.eg0_from(g, up, threads=2)
+            assert c[v] == TOP
+    assert fewer > 10
