@@ -4,8 +4,10 @@ Metric (one JSON line on stdout): edges relaxed per second to fixpoint on the si
 uniform game -- n = 2^22 vertices, owner uniform, out-degree exactly 8 distinct uniform
 successors (m = 2^25), weights uniform in [-100, 100], seed 2.  The game is generated,
 copied into HBM and loaded into the solver before the timed region.  One step is one full
-solve to the least fixpoint: transpose, initial frontier, rounds and TOP proofs.  `value`
-= edges relaxed by the solves / their wall time.
+solve to the least fixpoint: initial frontier, rounds and TOP proofs (the transpose is built
+only when a round goes sparse; on this game none does).  `value` = edges relaxed by the
+solves / their wall time.  After the timed solves the TOP set of the result is certified on
+the device and checked on the host (untimed).
 
 The line also carries
   roofline      the lift kernel on plain dense rounds of the same game (proofs off, so every
@@ -14,8 +16,12 @@ The line also carries
                 measured copy-kernel bandwidth, and the profiler's HBM bytes per launch
                 from the committed counter summary of this workload.
   cpu_baseline  the reference's own CPU solver (cpu_solver.c compiled from its sources into
-                oracle/_ref; the oracle's restatement if that build is absent), EG0 Jacobi
-                sweeps on this host, on a bounded number of sweeps of the same game.
+                oracle/_ref; the oracle's restatement if that build is absent) on this host:
+                EG0 Jacobi sweeps on every physical core available (OMP_PLACES=cores,
+                OMP_PROC_BIND=close) and EG, the worklist default, on one core, each on a
+                bounded sample of the same game; nproc/lscpu/affinity recorded.
+  skewed        the skewed-degree game (n = 2^20, power-law degrees): full solves and the
+                plain dense round's roofline.
   batched       4096 games of the batched family (seed 4) in one launch vs one game per
                 launch (single-game path, inputs resident) on a stratified sample.
   serving       a Poisson request stream of the serving family (seed 5) through the
@@ -61,32 +67,75 @@ def read_profile_traffic(workload_key):
     return float(mo.group(1)) if mo else None
 
 
-def cpu_baseline(g, budget_s=20.0):
-    """The reference CPU solver on a bounded sample of the same game: EG0 Jacobi sweeps
-    (cpu_solver.c:109-185; a sweep relaxes all m edges), OpenMP guided(4)."""
-    from oracle.oracle import Oracle, Reference, game_from_arrays
+def host_topology():
+    """CPUs this process may run on, the physical cores among them, the cgroup CPU quota,
+    and what nproc and lscpu report."""
+    import subprocess
+    aff = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in aff:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            cores.add((open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    lscpu = {}
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                lscpu[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=20).stdout.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        nproc = None
+    return {"nproc": nproc, "affinity_cpus": len(aff), "physical_cores_in_affinity": len(cores),
+            "cgroup_cpu_quota": quota, "lscpu": lscpu}
 
-    og = game_from_arrays(g.n, g.n0, g.ptr, g.succ, g.w)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    threads = min(threads, 16)
-    if Reference.available():
-        R = Reference()
-        kind = "reference"
-        run = lambda k: R.solve(og, "eg0", max_loop=k, threads=threads)[2]
-    else:
-        O = Oracle()
-        kind = "port"
 
-        def run(k):
-            t = time.perf_counter()
-            O.eg0(og, max_loop=k, threads=threads)
-            return time.perf_counter() - t
-    probe = run(2)
-    sweeps = max(2, int(budget_s / max(probe / 2, 1e-3)))
-    secs = run(sweeps)
-    return {"value": sweeps * g.m / secs, "unit": "edges relaxed/s", "cores": threads, "kind": kind,
-            "sample": f"EG0 (--cpu --eg0 {sweeps}): {sweeps} Jacobi sweeps over all {g.m} edges of the same game "
-                      f"from f=0, {secs:.2f} s, OpenMP guided(4), {threads} threads"}
+def cpu_baseline(g, budget_s=15.0):
+    """The reference's own CPU solver (src/cpu_solver.c compiled into oracle/_ref) on bounded
+    samples of the same game, each in its own process so the OpenMP placement applies:
+    EG0 Jacobi sweeps (src/cpu_solver.c:109-185) on every physical core this process may use
+    (OMP_PLACES=cores OMP_PROC_BIND=close, as script/runner_omp:30-31), and EG, the
+    reference's default worklist solver (src/cpu_solver.c:190-333), on one core."""
+    import subprocess
+    import tempfile
+    import egsolver_amd as E
+    topo = host_topology()
+    threads = topo["physical_cores_in_affinity"]
+    if topo["cgroup_cpu_quota"]:
+        threads = max(1, min(threads, int(topo["cgroup_cpu_quota"])))
+    script = os.path.join(ROOT, "oracle", "cpu_baseline.py")
+    res = {}
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        path = os.path.join(d, "game.egs")
+        E.save_csr(g, path)
+        for algo, t, budget in (("eg0", threads, budget_s), ("eg", 1, budget_s / 2)):
+            env = dict(os.environ, OMP_NUM_THREADS=str(t), OMP_PLACES="cores", OMP_PROC_BIND="close")
+            r = subprocess.run([sys.executable, script, "--game", path, "--algo", algo, "--threads", str(t),
+                                "--budget", str(budget)], capture_output=True, text=True, env=env, timeout=600)
+            if r.returncode != 0:
+                raise SystemExit(f"cpu baseline ({algo}) failed: {r.stderr[-2000:]}")
+            res[algo] = json.loads(r.stdout.strip().splitlines()[-1])
+            log(f"cpu baseline {algo}: {res[algo]['edges_per_s']:.3e} edges/s on {t} thread(s)")
+    eg0, eg = res["eg0"], res["eg"]
+    return {"value": eg0["edges_per_s"], "unit": "edges relaxed/s", "cores": threads, "kind": eg0["kind"],
+            "sample": eg0["sample"] + f" (OMP_PLACES=cores OMP_PROC_BIND=close; {threads} physical cores)",
+            "iteration_only_edges_per_s": eg0["edges_relaxed"] / eg0["iteration_s"],
+            "eg_one_core": {"value": eg["edges_per_s"], "iteration_only_value": eg["edges_per_s_iteration_only"],
+                            "unit": "edges relaxed/s", "cores": 1, "kind": eg["kind"], "sample": eg["sample"],
+                            "transpose_s": eg["transpose_s"]},
+            "host": topo}
 
 
 def single_game(E, g, args):
@@ -130,9 +179,18 @@ def single_game(E, g, args):
     t_all = time.perf_counter() - t_all
     vals = s.values()
     violations = s.violations()
-    s.close()
     if violations:
         raise SystemExit(f"fixpoint certificate failed: {violations} vertices")
+    # untimed: the TOP set of the last solve certified on the device and checked on the host
+    # (host fixpoint check plus the certificate's inequalities), independent of the proofs
+    from egsolver_amd import verify
+    t_c = time.perf_counter()
+    cert = verify.certify(s, g, vals)
+    cert["check_s"] = time.perf_counter() - t_c
+    s.close()
+    if not cert["ok"]:
+        raise SystemExit(f"TOP-set certificate failed: {cert}")
+    log(f"certificate of the TOP set: {cert['top_vertices']} vertices, {cert['passes']} passes, host check ok")
     edges = sum(st.edges_relaxed for _, st in solves)
     last = solves[-1][1]
     info = {"solve_ms_mean": 1e3 * t_all / args.steps, "solve_ms_min": 1e3 * min(t for t, _ in solves),
@@ -140,8 +198,58 @@ def single_game(E, g, args):
             "top_marked": last.top_marked, "top_vertices": int((vals == 2147483647).sum()),
             "value_checksum": int(np.bitwise_xor.reduce(vals.astype(np.uint32) * np.uint32(2654435761))),
             "fixpoint_violations": violations, "device_ms_prepare": last.ms_prepare,
-            "device_ms_iterate": last.ms_iterate}
+            "device_ms_iterate": last.ms_iterate, "transpose_built": bool(last.csc_built),
+            "top_set_certificate": cert}
     return edges / t_all, t_all, info, plain, shape
+
+
+def skewed_game(E, args):
+    """The skewed-degree game (n = 2^20, power-law out-degrees 1..10^5, exponent 2.1, weights
+    U[-100,100], seed 3): full solves to the least fixpoint, certified like the headline game,
+    plus the plain dense round (proofs off) of the same game for its roofline."""
+    g = E.config_game(3, scale=args.scale)
+    s = E.Solver(g.n, g.m)
+    s.load(g)
+    s.prepare(E.options(certify_every=-1))
+    _, a = s.iterate(args.warmup)
+    _, b = s.iterate(args.rounds // 4)
+    d = {k: getattr(b, k) - getattr(a, k) for k in ("processed", "edges_relaxed", "lifted", "pred_visits", "phases")}
+    ms_dev = b.ms_iterate - a.ms_iterate
+    per_round = ms_dev / max(d["phases"], 1)
+    abytes = algorithmic_bytes(d) / max(d["phases"], 1)
+    for _ in range(args.warmup_solves):
+        s.prepare(E.options())
+        s.iterate()
+    times, st = [], None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        s.prepare(E.options())
+        conv, st = s.iterate()
+        times.append(time.perf_counter() - t0)
+        if not conv:
+            raise SystemExit("skewed game: a solve did not reach the fixpoint")
+    vals = s.values()
+    from egsolver_amd import verify
+    cert = verify.certify(s, g, vals)
+    s.close()
+    if not cert["ok"]:
+        raise SystemExit(f"skewed game: TOP-set certificate failed: {cert}")
+    out = {"workload": "skewed-degree game (n=2^20, power-law out-degrees 1..1e5 exponent 2.1, weights U[-100,100], "
+                       "seed 3); step = one full solve to the least fixpoint",
+           "n": g.n, "m": g.m, "max_outdeg": int(np.diff(g.ptr).max()), "mg": g.mg,
+           "solve_ms_mean": 1e3 * sum(times) / len(times), "solve_ms_min": 1e3 * min(times),
+           "edges_relaxed_per_s": st.edges_relaxed * len(times) / sum(times), "rounds_per_solve": st.phases,
+           "proofs": st.certificates, "transpose_built": bool(st.csc_built), "device_ms_prepare": st.ms_prepare,
+           "device_ms_iterate": st.ms_iterate, "top_vertices": cert["top_vertices"],
+           "top_set_certificate_ok": cert["ok"],
+           "plain_rounds": {"rounds": d["phases"], "device_ms_per_round": per_round,
+                            "algorithmic_bytes_per_round": abytes,
+                            "edges_per_s": d["edges_relaxed"] / (ms_dev * 1e-3),
+                            "achieved_gbps": abytes / (per_round * 1e-3) / 1e9,
+                            "frac_of_hbm_peak": abytes / (per_round * 1e-3) / 1e9 / HBM_PEAK_GBPS}}
+    log(f"skewed game: m {g.m}, solve {out['solve_ms_mean']:.2f} ms, plain round {per_round:.3f} ms "
+        f"({out['plain_rounds']['achieved_gbps']:.0f} GB/s)")
+    return out
 
 
 def gen_many(E, family, seed, count, scale_n=1.0, workers=16):
@@ -340,6 +448,7 @@ def main():
     ap.add_argument("--certify-every", type=int, default=0, help="rounds between proof attempts (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-skewed", action="store_true", help="skip the skewed-degree game (n=2^20) leg")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-games", type=int, default=4096)
     ap.add_argument("--plain-games", type=int, default=128)
@@ -373,7 +482,7 @@ def main():
         "unit": "edges relaxed/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup_solves,
+        "warmup": args.warmup,
         "ms_per_step": 1e3 * t_all / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
@@ -382,8 +491,12 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "single large uniform game (n=2^22, out-degree 8, weights U[-100,100], seed 2); "
-                        "step = one full solve to the least fixpoint (transpose, rounds, TOP proofs)",
-            "n": g.n, "m": g.m, "mg": g.mg, "shape": shape, **info,
+                        "step = one full solve to the least fixpoint (initial frontier, rounds, TOP proofs; "
+                        "the transpose is built only if a round goes sparse)",
+            "n": g.n, "m": g.m, "mg": g.mg, "shape": shape,
+            "warmup_note": "warmup = untimed plain rounds before the roofline rounds; "
+                           f"{args.warmup_solves} untimed full solve(s) precede the timed ones",
+            "warmup_solves": args.warmup_solves, **info,
             "plain_rounds": plain,
         },
         "roofline": {
@@ -397,6 +510,8 @@ def main():
     }
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(g, args.cpu_budget)
     del g
+    if not args.no_skewed:
+        line["skewed"] = skewed_game(E, args)
     if not args.no_batch:
         line["batched"], _ = batched(E, args)
     if not args.no_serve:

@@ -63,6 +63,8 @@ class EgsStats(ctypes.Structure):
         ("cert_passes", ctypes.c_int64),
         ("top_marked", ctypes.c_int64),
         ("cert_pruned", ctypes.c_int64),
+        ("csc_built", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
     def as_dict(self):

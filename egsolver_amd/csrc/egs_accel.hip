@@ -719,7 +719,7 @@ hipError_t certify_prefetch(const DevGame &g, const CertBuffers &cb, int wmax, h
 }
 
 int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int cap, int max_cands, bool has_big,
-                bool hist_ready, hipStream_t st, CertResult &res) {
+                bool hist_ready, const volatile uint32_t *stop, hipStream_t st, CertResult &res) {
     constexpr int kBigGrid = 512;           // workgroups of the kernels for vertices above big_deg
     res = CertResult{};
     if (wmax <= 0 || !g.rise) return 0;
@@ -751,6 +751,7 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
     while (G < 16 && 2 * G < (double)g.m / g.n) G <<= 1;
     const int bf_grid = grid_for((int64_t)g.n * G, 256, 4096);
     for (int c = 0; c < ncand; c++) {
+        if (stop && *stop) return 0;        // the run is being stopped: no new candidate
         const int B = (int)cands[c];
         res.threshold = B;
         res.tried++;
@@ -777,6 +778,7 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
         int passes = 0;
         bool converged = false, diverged = false, rejected = false;
         while (!converged && !diverged && !rejected && passes < cap) {
+            if (stop && *stop) return 0;    // abandon the candidate: nothing was marked
             const int upto = std::min(cap, queued == 1 ? first_chunk : queued + chunk);
             for (int p = queued; p < upto; p++) {
                 launch_bf_pass(G, bf_grid, st, g, cb.h, bound, bm[(p - 1) % 3], bm[p % 3], bm[(p + 1) % 3], p, cb.bfcnt,
@@ -901,7 +903,8 @@ int certify_all_top(const DevGame &g, const CertBuffers &cb, int wmax, int max_p
             if (cnt[p % kCntSlots] == 0) { converged = true; break; }
         }
         queued = upto;
-        chunk = std::min(chunk * 2, 256);
+        // a chunk must stay below the counter ring: pass p writes slot p and zeroes slot p + 1
+        chunk = std::min(chunk * 2, kCntSlots / 2);
     }
     res.marked = fl[3];                     // TOP vertices
     res.closure_failed = fl[0];             // TOP vertices that left the certified set

@@ -74,6 +74,7 @@ constexpr int kSparseWantRounds = 6;
 constexpr int kCertFirstCap = 64;     // longest-path passes per candidate at the first attempt
 constexpr int kCertCandidates = 8;    // gap thresholds tried per attempt (plus the last-round rise set)
 constexpr int kCertEvery = 8;         // default rounds before the first proof attempt
+constexpr int kStopBatch = 4;         // rounds queued ahead of the host when a stop flag is polled
 
 int pow2ceil(int x) {
     int p = 1;
@@ -429,6 +430,9 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     sh.wave_grid = 1024;
     sh.block_grid = 1024;
     s->batch_rounds = o->batch_rounds > 0 ? o->batch_rounds : 32;
+    // with a stop flag at most kStopBatch rounds are queued ahead of the host, so a stop
+    // takes effect within a few rounds
+    if (o->stop_flag) s->batch_rounds = std::min(s->batch_rounds, kStopBatch);
     s->phase_cap = egs_max_loop_c(g.m, g.n, g.mg, o->max_phases);
     s->stop_flag = o->stop_flag;
     s->cert_every = o->certify_every == 0 ? kCertEvery : (o->certify_every < 0 ? 0 : o->certify_every);
@@ -520,6 +524,7 @@ static int collect_stats(egs_solver *s) {
     s->st.lifted = (int64_t)h[2];
     s->st.pred_visits = (int64_t)h[3];
     s->st.phases = s->h_probe[kPinRounds];
+    s->st.csc_built = s->csc_built ? 1 : 0;
     s->st.converged = list_total(s, s->h_probe + kPinConv) == 0;
     return EGS_OK;
 }
@@ -548,9 +553,10 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
             if (hist) HIP_TRY(egs::certify_prefetch(s->g, cb, s->wmax, s->stream));
             HIP_TRY(hipStreamSynchronize(s->stream));
             if (list_total(s, s->h_probe + kPinConv) == 0) break;
+            if (s->stop_flag && *s->stop_flag) break;
             egs::CertResult cr;
             const int got = egs::certify_top(s->g, cb, s->wmax, (int)(s->next_round - 1), s->cert_cap,
-                                             kCertCandidates, s->sh.use_wave != 0, hist, s->stream, cr);
+                                             kCertCandidates, s->sh.use_wave != 0, hist, s->stop_flag, s->stream, cr);
             if (got < 0) return fail(EGS_E_HIP, "certification: %s", hipGetErrorString(hipGetLastError()));
             s->st.cert_attempts++;
             s->st.cert_passes += cr.passes;
@@ -574,11 +580,16 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
             pending = -1;
         }
         if (accel) nb = (int)std::max<int64_t>(1, std::min<int64_t>(nb, s->next_cert - (s->next_round - 1)));
-        for (int k = 0; k < nb; k++) {
+        int k = 0;
+        for (; k < nb; k++) {
+            // the stop flag is polled before every round, as the reference polls
+            // timeout_expired once per round (src/dev_EG_alg.cu:376)
+            if (s->stop_flag && *s->stop_flag) break;
             HIP_TRY(egs::launch_round(s->g, s->sh, (int)s->next_round, s->stream));
             s->next_round++;
         }
-        launched += nb;
+        if (k == 0) break;
+        launched += k;
         const int slot = (pending + 1) & 1;
         HIP_TRY(hipMemcpyAsync(s->h_probe + kPinProbe + slot * 128,
                                s->d_lcnt + (s->next_round % 3) * egs::kListCounters,

@@ -32,6 +32,7 @@ extern char *nodeOwner, *nodeFlags;
 extern char **nodeName;
 
 static egs_solver *g_solver = NULL;
+int egs_ref_certify_every = 0;
 
 static void die(const char *what) {
     fprintf(stderr, "ERROR: %s : %s\n", what, egs_last_error());
@@ -91,6 +92,7 @@ void gpu_solver(void) {
     opt.max_phases = aggiorna_max_loop((long)num_archi, (long)num_nodi, (long)MG_pesi,
                                        configuration.max_loop_val);
     opt.stop_flag = (const volatile uint32_t *)&timeout_expired;
+    opt.certify_every = egs_ref_certify_every;
     printf("Running EG on GPU. (MG_pesi=%d max_loop=%ld num nodes=%d max weight=%d)\n", MG_pesi,
            (long)opt.max_phases, num_nodi, max_pesi);
     fflush(stdout);

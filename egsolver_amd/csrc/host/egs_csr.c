@@ -6,11 +6,13 @@
  * loads again at file-read speed.  Layout, little-endian, int32 arrays:
  *   64-byte header: magic "EGSCSR1", version, header size, n, n0, m, MG, max |w|, checksum
  *   ptr[n+1]  succ[m]  w[m]  ext_of_int[n]
- * The checksum covers the four arrays as one sequence of 32-bit words: each block of
- * kBlockWords words is hashed on its own (so blocks hash in parallel) and the block hashes
- * are folded in order.  A load checks the header against the file size, the checksum and
- * the CSR itself (offsets from 0 to m, non-decreasing, an edge per vertex, successors in
- * range, external ids in range), so a damaged or foreign file is refused.
+ * The checksum covers the header fields n, n0, m, MG and max |w| and the four arrays as one
+ * sequence of 32-bit words: each block of kBlockWords words is hashed on its own (so blocks
+ * hash in parallel) and the block hashes are folded in order, then the header fields.  A
+ * load checks the header against the file size, the checksum, the CSR itself (offsets from
+ * 0 to m, non-decreasing, an edge per vertex, successors in range, external ids in range)
+ * and recomputes MG and max |w| from the weights, so a damaged or foreign file is refused
+ * (a wrong n0 or MG would otherwise change every value the solver returns).
  */
 #include <limits.h>
 #include <stdarg.h>
@@ -21,7 +23,7 @@
 #include "egs_host.h"
 
 #define EGS_CSR_MAGIC "EGSCSR1"
-#define EGS_CSR_VERSION 1u
+#define EGS_CSR_VERSION 2u
 
 typedef struct {
     char magic[8];
@@ -75,6 +77,27 @@ static uint64_t words_hash(const int32_t *a, int64_t na, const int32_t *b, int64
     return H;
 }
 
+/* The checksum: the arrays' hash folded with the header fields it depends on. */
+static uint64_t file_hash(const csr_header *h, const int32_t *ptr, const int32_t *succ, const int32_t *w,
+                          const int32_t *ext_of_int) {
+    uint64_t H = words_hash(ptr, (int64_t)h->n + 1, succ, h->m, w, h->m, ext_of_int, h->n);
+    H = mix64(H ^ (uint64_t)(uint32_t)h->n);
+    H = mix64(H ^ (uint64_t)(uint32_t)h->n0);
+    H = mix64(H ^ (uint64_t)h->m);
+    H = mix64(H ^ (uint64_t)(uint32_t)h->mg);
+    return mix64(H ^ (uint64_t)(uint32_t)h->max_abs_w);
+}
+
+static int32_t max_abs(const int32_t *w, int64_t m) {
+    int32_t mx = 0;
+#pragma omp parallel for schedule(static) reduction(max : mx) if (m > (1 << 16))
+    for (int64_t e = 0; e < m; e++) {
+        const int32_t a = w[e] < 0 ? (w[e] == INT32_MIN ? INT32_MAX : -w[e]) : w[e];
+        if (a > mx) mx = a;
+    }
+    return mx;
+}
+
 int egs_csr_save(const char *path, int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *ptr,
                  const int32_t *succ, const int32_t *w, const int32_t *ext_of_int, char *errbuf, int errlen) {
     if (!path || n < 1 || n0 < 0 || n0 > n || m < 0 || m > INT32_MAX || !ptr || !succ || !w || !ext_of_int) {
@@ -90,14 +113,8 @@ int egs_csr_save(const char *path, int32_t n, int32_t n0, int64_t m, int32_t mg,
     h.n0 = n0;
     h.m = m;
     h.mg = mg;
-    int32_t mx = 0;
-#pragma omp parallel for schedule(static) reduction(max : mx) if (m > (1 << 16))
-    for (int64_t e = 0; e < m; e++) {
-        const int32_t a = w[e] < 0 ? (w[e] == INT32_MIN ? INT32_MAX : -w[e]) : w[e];
-        if (a > mx) mx = a;
-    }
-    h.max_abs_w = mx;
-    h.checksum = words_hash(ptr, (int64_t)n + 1, succ, m, w, m, ext_of_int, n);
+    h.max_abs_w = max_abs(w, m);
+    h.checksum = file_hash(&h, ptr, succ, w, ext_of_int);
     FILE *f = fopen(path, "wb");
     if (!f) { cerr(errbuf, errlen, "cannot create %s", path); return -1; }
     int ok = fwrite(&h, sizeof(h), 1, f) == 1 && fwrite(ptr, 4, (size_t)n + 1, f) == (size_t)n + 1 &&
@@ -158,7 +175,7 @@ int egs_csr_load(const char *path, egs_csr *out, char *errbuf, int errlen) {
              fread(out->w, 4, (size_t)m, f) == (size_t)m && fread(out->ext_of_int, 4, (size_t)n, f) == (size_t)n;
     fclose(f);
     if (!ok) { egs_csr_free(out); cerr(errbuf, errlen, "read error on %s", path); return -1; }
-    if (words_hash(out->ptr, n + 1, out->succ, m, out->w, m, out->ext_of_int, n) != h.checksum) {
+    if (file_hash(&h, out->ptr, out->succ, out->w, out->ext_of_int) != h.checksum) {
         egs_csr_free(out);
         cerr(errbuf, errlen, "%s: checksum mismatch (damaged file)", path);
         return -1;
@@ -171,6 +188,12 @@ int egs_csr_load(const char *path, egs_csr *out, char *errbuf, int errlen) {
     if (bad) {
         egs_csr_free(out);
         cerr(errbuf, errlen, "%s: the stored graph is not a valid game", path);
+        return -1;
+    }
+    int32_t mg = 0;
+    if (egs_compute_mg(h.n, out->ptr, out->w, &mg) != 0 || mg != h.mg || max_abs(out->w, m) != h.max_abs_w) {
+        egs_csr_free(out);
+        cerr(errbuf, errlen, "%s: MG or max |w| in the header does not match the weights", path);
         return -1;
     }
     out->n = h.n;

@@ -106,6 +106,7 @@ static void usage(void) {
     fprintf(stderr, "  --shuffling N\n\tLanes per low-degree vertex, N=1,2,4,8,16,32 (default: chosen from the degrees).\n");
     fprintf(stderr, "  --timeout [S]\n\tStop after about S seconds with partial values (default S=%d).\n",
             DEFAULT_TIMEOUT_SEC);
+    fprintf(stderr, "  --no-proofs\n\tPlain value iteration: no proofs that climbing vertices are TOP (same values, many more rounds).\n");
     fprintf(stderr, "  --save-csr FILE\n\tAfter reading, save the game in its internal numbering as a binary cache.\n");
     fprintf(stderr, "  --load-csr FILE\n\tRead the game from a binary cache made by --save-csr instead of the input.\n");
     fprintf(stderr, "\n Output:\n");
@@ -173,6 +174,8 @@ static void setconfig(int argc, char **argv) {
             configuration.computationType = CPU_COMPUTATION;
         } else if (!strcmp(a, "--gpu")) {
             configuration.computationType = GPU_COMPUTATION;
+        } else if (!strcmp(a, "--no-proofs")) {
+            egs_ref_certify_every = -1;
         } else if (!strcmp(a, "--printdegrees")) {
         } else if (!strcmp(a, "--onelineout")) {
             configuration.onelineout = YES_ONELINEOUT;

@@ -54,9 +54,12 @@ typedef struct egs_options {
                                out of passes); 0 = auto (1024) */
     int64_t max_phases;     /* phase cap; < 0 = the reference bound n*m*MG+1
                                (src/egsolver.c:64-90) */
-    const volatile uint32_t *stop_flag;  /* polled between batches; non-zero stops the
-                               run early with partial values (the reference's
-                               timeout_expired, src/sig_handling.c:57-64) */
+    const volatile uint32_t *stop_flag;  /* polled before every round and before each step
+                               of a proof attempt; non-zero stops the run early with
+                               partial values (the reference's timeout_expired,
+                               src/sig_handling.c:57-64, polled once per round at
+                               src/dev_EG_alg.cu:376).  With a stop flag at most 4 rounds
+                               are queued ahead of the host. */
     int32_t slices;         /* dense rounds in successor-range phases (their value gathers
                                stay in L2): 0 = auto (4 phases when n >= 2^20 and the solver
                                was created with n_cap >= 2^20), 1 = one sweep, > 1 = phases */
@@ -78,6 +81,9 @@ typedef struct egs_stats {
     int64_t cert_passes;    /* longest-path passes spent on proofs */
     int64_t top_marked;     /* vertices set to TOP by proofs */
     int64_t cert_pruned;    /* proof candidates that lost vertices to the closure check */
+    int32_t csc_built;      /* 1 when this solve built the transpose (only sparse rounds
+                               need it; dense-only solves never build it) */
+    int32_t reserved;
 } egs_stats;
 
 /* ---- device ---------------------------------------------------------------------- */

@@ -98,6 +98,11 @@ int copia_dati_su_device(void);
  * (the reference's --shuffling K, src/egsolver.c:699-731); timeout_expired stops the
  * run with partial values. */
 void gpu_solver(void);
+/* No reference counterpart.  Rounds between attempts to prove climbing vertices TOP,
+ * passed to egs_options.certify_every by gpu_solver: 0 (the default) = the library's
+ * default, < 0 = never (plain value iteration, as the reference runs).  The driver's
+ * --no-proofs sets it to -1. */
+extern int egs_ref_certify_every;
 /* Defined by the driver (src/egsolver.c:64-90); used here for the phase cap. */
 long aggiorna_max_loop(long Narchi, long Nnodi, long MGpesi, long maxloop);
 

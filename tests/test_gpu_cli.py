@@ -84,3 +84,37 @@ def test_cli_round_cap_is_under_approximation():
     assert part.keys() == full.keys()
     assert all(part[k] <= full[k] for k in full)
     assert any(part[k] < full[k] for k in full)
+
+
+def _ladder_text(N):
+    """script/multi/mkladder.c:42-51: even ids MAX with 0-weight edges to i+1, i+2; odd ids
+    MIN with edges to i+1 (0) and i+2 (-1), indices mod 2N.  Least fixpoint: even 0, odd TOP;
+    plain value iteration needs about N rounds (the odd cycle climbs 1 per round to MG = N)."""
+    M = 2 * N
+    out = [f"{M - 1};"]
+    out += [f"{i} MAX {(i + 1) % M}:0,{(i + 2) % M}:0;" for i in range(0, M, 2)]
+    out += [f"{i + 1} MIN {(i + 2) % M}:0,{(i + 3) % M}:-1;" for i in range(0, M, 2)]
+    return "\n".join(out) + "\n"
+
+
+def test_cli_timeout_stops_with_partial_values(tmp_path):
+    """--timeout 2 (SIGALRM sets timeout_expired, src/sig_handling.c:57-64) on a ladder of
+    10^6 rungs solved by plain value iteration (--no-proofs, about 10^6 rounds): the run
+    stops after about 2 s and prints values pointwise at most the least fixpoint."""
+    N = 1_000_000
+    path = str(tmp_path / "ladder.mpg")
+    with open(path, "w") as f:
+        f.write(_ladder_text(N))
+    _say(f"cli --timeout 2 --no-proofs ladder N={N}")
+    out = _run("--input", path, "--no-proofs", "--timeout", "2")
+    assert "Setting 2 seconds timeout" in out
+    secs = float(re.search(r"Solving time: ([0-9.]+) sec", out).group(1))
+    assert 1.5 < secs < 6.0, secs
+    vals = _values(out)
+    assert len(vals) == 2 * N
+    assert all(vals[e] == 0 for e in range(0, 2 * N, 2))
+    odd = [vals[e] for e in range(1, 2 * N, 2)]
+    assert max(odd) < TOP and max(odd) > 0          # partial: climbing, not yet TOP
+    # with proofs (the default) the same game solves to the least fixpoint at once
+    full = _values(_run("--input", path))
+    assert all(full[e] == (0 if e % 2 == 0 else TOP) for e in range(2 * N))

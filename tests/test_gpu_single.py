@@ -344,3 +344,51 @@ def test_top_certificate_small_games(oracle):
                 found, h, _ = s.top_certificate(max_passes=4096)
                 code, _ = verify.check_top_certificate(g, bad, h)
                 assert not found and code != 0
+
+
+def test_stop_flag_from_another_thread_then_resume(tmp_path):
+    """The stop flag (the reference's timeout_expired, polled once per round at
+    src/dev_EG_alg.cu:376) set by another host thread 0.5 s into a plain value iteration
+    (proofs off) of a ladder of 300,000 rungs (about 300,000 rounds): the run stops within a
+    few rounds with values pointwise at most the least fixpoint (even 0, odd TOP,
+    script/multi/mkladder.c), then resumes to it."""
+    import ctypes
+    import threading
+    import time
+    import egsolver_amd as E
+    from test_gpu_cli import _ladder_text
+    N = 300_000
+    path = str(tmp_path / "ladder.mpg")
+    with open(path, "w") as f:
+        f.write(_ladder_text(N))
+    g = E.read_mpg(path)
+    lfp = np.where(g.ext_of_int % 2 == 0, 0, TOP).astype(np.int32)
+    _say(f"stop flag, ladder N={N}, n {g.n}")
+    flag = ctypes.c_uint32(0)
+    with E.Solver(g.n, g.m) as s:
+        s.load(g)
+        s.prepare(E.options(certify_every=-1, stop_flag=flag))
+        t_set = []
+
+        def fire():
+            t_set.append(time.perf_counter())
+            flag.value = 1
+        timer = threading.Timer(0.5, fire)
+        t0 = time.perf_counter()
+        timer.start()
+        conv, st = s.iterate()
+        t1 = time.perf_counter()
+        timer.join()
+        assert not conv and t_set, "the run ended before the stop flag was set"
+        lag = t1 - t_set[0]
+        _say(f"stopped after {st.phases} rounds, {1e3 * lag:.2f} ms after the flag")
+        assert lag < 0.05, lag
+        part = s.values()
+        assert (part <= lfp).all()
+        assert part[g.ext_of_int % 2 == 1].max() > 0
+        flag.value = 0
+        t2 = time.perf_counter()
+        conv, st2 = s.iterate()
+        _say(f"resumed: {st2.phases} rounds in total, {time.perf_counter() - t2:.2f} s more")
+        assert conv and st2.phases > st.phases
+        assert np.array_equal(s.values(), lfp)

@@ -157,6 +157,10 @@ def test_csr_cache_refuses_damaged_files(tmp_path):
         "truncated": raw[:-4],
         "magic": b"XGSCSR1\0" + raw[8:],
         "longer": raw + b"\0\0\0\0",
+        # header fields (64-byte header: n at 16, n0 at 20, m at 24, MG at 32, max |w| at 36)
+        "n0": raw[:20] + bytes([raw[20] ^ 1]) + raw[21:],
+        "mg": raw[:32] + bytes([raw[32] ^ 1]) + raw[33:],
+        "max_abs_w": raw[:36] + bytes([raw[36] ^ 2]) + raw[37:],
     }
     for name, data in cases.items():
         q = str(tmp_path / f"{name}.egs")
@@ -166,6 +170,20 @@ def test_csr_cache_refuses_damaged_files(tmp_path):
             E.load_csr(q)
     with pytest.raises(ValueError):
         E.load_csr(str(tmp_path / "absent.egs"))
+
+
+def test_csr_cache_refuses_wrong_mg(tmp_path):
+    """A file written with an MG that does not follow from its weights (the checksum then
+    covers the wrong value) is refused on load: MG and max |w| are recomputed."""
+    g = E.config_game(1)
+    bad = G.Game(n=g.n, n0=g.n0, ptr=g.ptr, succ=g.succ, w=g.w, mg=g.mg + 1, ext_of_int=g.ext_of_int)
+    p = str(tmp_path / "mg.egs")
+    E.save_csr(bad, p)
+    with pytest.raises(ValueError, match="MG"):
+        E.load_csr(p)
+    E.save_csr(g, p)
+    h = E.load_csr(p)
+    assert (h.n, h.n0, h.m, h.mg) == (g.n, g.n0, g.m, g.mg)
 
 
 def test_csr_cache_refuses_invalid_graph(tmp_path):

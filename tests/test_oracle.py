@@ -361,3 +361,22 @@ def test_eg0_from_seeded(oracle):
             c, _ = oracle.eg0_from(g, up, threads=2)
             assert c[v] == TOP
     assert fewer > 10
+
+
+@pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref not built (needs the reference tree)")
+def test_reference_eg0_race_free_at_bench_threads(oracle):
+    """The reference's EG0 declares its inner edge index `idy` at function scope
+    (src/cpu_solver.c:111) and uses it inside `#pragma omp parallel for` (:146-149), where it
+    is shared between threads: a data race.  The compiled build keeps it in a register, so
+    the values still come out right; this checks that at the bench's thread count (16) on a
+    game of 2^20 edges, against the restatement, which has no shared index."""
+    import egsolver_amd as E
+    g = E.config_game(2, scale=1 / 32)                     # n = 131072, m = 2^20
+    og = game_from_arrays(g.n, g.n0, g.ptr, g.succ, g.w)
+    assert og.m == 1 << 20
+    R = Reference()
+    for cap in (7, 40):
+        a, pa, _ = R.solve(og, "eg0", max_loop=cap, threads=16)
+        b, sb = oracle.eg0(og, max_loop=cap, threads=16)
+        assert pa == sb * og.n
+        assert np.array_equal(a, b), cap
