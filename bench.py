@@ -453,7 +453,7 @@ def main():
     ap.add_argument("--batch-games", type=int, default=4096)
     ap.add_argument("--plain-games", type=int, default=128)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--serve-capacity-requests", type=int, default=500)
     ap.add_argument("--serve-plain-requests", type=int, default=300)
     ap.add_argument("--serve-load", type=float, default=0.75)

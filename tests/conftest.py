@@ -68,3 +68,42 @@ def exact_by_certificate(oracle, g, vals, solver):
     assert np.array_equal(vals, ref), f"{int((vals != ref).sum())} values differ from the oracle's seeded EG0"
     rep["oracle_sweeps"] = int(sweeps)
     return rep
+
+
+def gen_family(family, seed, count, workers=16):
+    """Games 0..count-1 of a multi-game family (batched: 4, serving: 5), generated in parallel."""
+    from concurrent.futures import ThreadPoolExecutor
+    import egsolver_amd as E
+
+    def one(i):
+        return E.generate(E.family_params(family, seed, i), stream=i)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(one, range(count)))
+
+
+def exact_sample(oracle, games, results, idx, solver, unseeded_max_n=12000, threads=16):
+    """Bit-exact check of results[i] for i in idx: the oracle's EG0 from 0 (the reference's
+    algorithm end to end) on games up to unseeded_max_n vertices, and on every sampled game
+    the certificate route of exact_by_certificate (host fixpoint check, device certificate of
+    the TOP set checked on the host, oracle EG0 from the certified TOP set).  `solver` must
+    be sized for the largest sampled game.  Returns (games checked from 0, games checked
+    through the certificate)."""
+    import numpy as np
+    from egsolver_amd import verify
+    unseeded = seeded = 0
+    for i in idx:
+        g, vals = games[i], results[i]
+        og = to_oracle_game(g)
+        if g.n <= unseeded_max_n:
+            ref, _ = oracle.eg0(og, threads=threads)
+            assert np.array_equal(vals, ref), (i, g.n, int((vals != ref).sum()))
+            unseeded += 1
+        solver.load(g)
+        solver.prepare()
+        rep = verify.certify(solver, g, vals)
+        assert rep["ok"], (i, g.n, rep)
+        start = np.where(vals == 2147483647, 2147483647, 0).astype(np.int32)
+        ref, _ = oracle.eg0_from(og, start, threads=threads)
+        assert np.array_equal(vals, ref), (i, g.n, int((vals != ref).sum()))
+        seeded += 1
+    return unseeded, seeded

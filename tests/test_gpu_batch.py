@@ -148,3 +148,42 @@ def test_batch_proof_fallback_games(oracle):
     for g, ref in zip(games, refs):
         alone, st = E.solve(g)
         assert np.array_equal(alone, ref)
+
+
+def test_config4_full_launch(oracle):
+    """The batched workload at its size: 4,096 games of the batched family (seed 4; n
+    log-uniform 1e3..5e4, out-degree 2..16, W log-uniform 10..1000, every 16th a complete
+    digraph of 200..500 vertices) in one launch.  Every game: the host fixpoint check and a
+    certificate of its TOP set checked on the host.  A sample of 40 games stratified by size,
+    with complete digraphs and games above the 40,448-vertex LDS limit: bit-exact against
+    the oracle (EG0 from 0 up to 12,000 vertices, EG0 from the certified TOP set on all)."""
+    import egsolver_amd as E
+    from conftest import exact_sample, gen_family
+    from egsolver_amd import verify
+    games = gen_family(E.games.FAMILY_BATCH, 4, 4096)
+    _say(f"config 4: {len(games)} games, {sum(g.n for g in games)} vertices, {sum(g.m for g in games)} edges")
+    with E.Batch() as b:
+        for g in games:
+            b.add(g)
+        b.upload()
+        st = b.solve()
+        assert st["converged"] == len(games)
+        vals = [v.copy() for v in b.values_view()]
+    _say(f"config 4 launch: {st['ms']:.1f} ms, {st['certificates']} proofs")
+    nmax, mmax = max(g.n for g in games), max(g.m for g in games)
+    with E.Solver(nmax, mmax) as s:
+        for i, g in enumerate(games):
+            viol, first = verify.check_fixpoint(g, vals[i])
+            assert viol == 0, (i, g.n, first)
+            s.load(g)
+            s.prepare()
+            rep = verify.certify(s, g, vals[i])
+            assert rep["ok"], (i, g.n, rep)
+        order = np.argsort([g.n for g in games], kind="stable")
+        idx = set(order[np.linspace(0, len(games) - 1, 32).round().astype(int)].tolist())
+        idx |= {i for i in range(15, len(games), 16)[:4]}                 # complete digraphs
+        idx |= set(order[-4:].tolist())                                   # the largest games
+        assert any(games[i].n > 40448 for i in idx)
+        unseeded, seeded = exact_sample(oracle, games, vals, sorted(idx), s)
+    _say(f"config 4 sample: {unseeded} games exact against EG0 from 0, {seeded} through the certificate")
+    assert unseeded >= 16 and seeded == len(idx)
