@@ -205,39 +205,60 @@ __global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, lon
 
 
 // k_closure over the phased layout (games whose dense rounds run in successor-range
-// phases): phase s folds each member's edges into slice s, so its value gathers come from
-// one slice that stays in L2.  The fold is the one of k_closure (an OR, an AND and a max or
-// min, so the order of the edges does not matter); between phases a member's partial best
-// is kept in h[v] and its flags in acc[v] (bit 0: an edge reaches D or TOP, bit 1: an edge
-// leaves D and does not reach TOP).  Membership depends only on values and rise stamps,
-// which no phase changes.  The last phase completes the fold and then does what k_closure
-// does after its edge loop, for every vertex.
-constexpr int kClosurePhaseU = 2;   // edges in flight per thread (about two per slice)
+// phases), in one launch shaped like k_dense_fused: workgroup b takes the tiles b, b + W,
+// ... and walks the slices in order, so the value gathers of the moment come from one slice
+// that stays in L2.  The fold is the one of k_closure (an OR, an AND and a max or min, so
+// the order of the edges does not matter); a member's partial best and its flags (bit 0:
+// an edge reaches D or TOP, bit 1: an edge leaves D and does not reach TOP; bit 2: a member
+// whose edges are in the slices) stay in LDS for the whole launch.  Membership depends only
+// on values and rise stamps, which this kernel does not change.  After the last slice it
+// does what k_closure does after its edge loop, for every vertex.
+constexpr int kClosurePhaseU = 2;   // edges in flight per thread
+
+__device__ __forceinline__ int wave_excl_scan_a(int x) {
+    const int lane = threadIdx.x & 63;
+    int inc = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += y;
+    }
+    return inc - x;
+}
 
 template <bool PACK>
-__global__ __launch_bounds__(256) void k_closure_phase(DevGame g, int B, int last, long long bound, int s,
+__global__ __launch_bounds__(256) void k_closure_fused(DevGame g, int B, int last, long long bound,
                                                        long long *__restrict__ h, unsigned *__restrict__ bits,
                                                        unsigned *__restrict__ clear, int *__restrict__ cnt,
                                                        int *__restrict__ flags) {
+    extern __shared__ long long lbest[];          // [tile][kSliceTile] partial bests, then the flags
     const long long scale = (long long)g.n + 1;
-    const int lane = threadIdx.x & 63;
-    const bool final_phase = s == g.phases - 1;
-    int members = 0, changed = 0, removed = 0;
-    // the stride is a multiple of 64, so a wave always covers 64 aligned vertices (two words)
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
-        const int x = g.val[v];
-        const bool member = in_d_x(g, (int)v, x, B, last);
-        const bool small = g.ptr[v + 1] - g.ptr[v] <= g.big_deg;
-        const bool owner1 = v >= g.n0;
-        long long best = owner1 ? LLONG_MAX : LLONG_MIN;
-        int fl = 0;
-        if (member && small) {
-            if (s > 0) {
-                best = h[v];
-                fl = g.acc[v];
-            }
-            const size_t i = slice_seg(v, s);
-            const int b = g.sptr[i], e = g.sptr[i + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int P = g.phases;
+    const int64_t tiles = ((int64_t)g.n + kSliceTile - 1) / kSliceTile;
+    const int64_t W = gridDim.x;
+    const int K = blockIdx.x < tiles ? (int)((tiles - blockIdx.x + W - 1) / W) : 0;
+    int *lfl = (int *)(lbest + (size_t)K * kSliceTile);
+    for (int k = 0; k < K; k++) {
+        const int64_t v = (blockIdx.x + k * W) * kSliceTile + threadIdx.x;
+        bool member = false;
+        if (v < g.n) member = in_d_x(g, (int)v, g.val[v], B, last) && g.ptr[v + 1] - g.ptr[v] <= g.big_deg;
+        lfl[k * kSliceTile + threadIdx.x] = member ? 4 : 0;
+        lbest[k * kSliceTile + threadIdx.x] = v >= g.n0 ? LLONG_MAX : LLONG_MIN;
+    }
+    for (int s = 0; s < P; s++) {
+        for (int k = 0; k < K; k++) {
+            const int64_t T = blockIdx.x + k * W;
+            const size_t seg = (size_t)T * P + s;
+            const int i = k * kSliceTile + threadIdx.x;
+            int fl = lfl[i];
+            // every lane's length counts in the prefix: the segments are laid out for all vertices
+            const int len = g.slen[seg * kSliceTile + threadIdx.x];
+            const int b = g.sbase[seg * kSliceWaves + wave] + wave_excl_scan_a(len);
+            const int e = b + len;
+            if (!(fl & 4) || len == 0) continue;
+            const bool owner1 = T * kSliceTile + threadIdx.x >= g.n0;
+            long long best = lbest[i];
             for (int k0 = b; k0 < e; k0 += kClosurePhaseU) {
                 int u[kClosurePhaseU], w[kClosurePhaseU], xu[kClosurePhaseU];
 #pragma unroll
@@ -259,40 +280,49 @@ __global__ __launch_bounds__(256) void k_closure_phase(DevGame g, int B, int las
                     }
                 }
             }
-            if (!final_phase) {
-                h[v] = best;
-                g.acc[v] = fl;
-            }
+            lbest[i] = best;
+            lfl[i] = fl;
         }
-        if (!final_phase) continue;
-        long long hv;
+    }
+    int members = 0, changed = 0, removed = 0;
+    for (int k = 0; k < K; k++) {
+        const int64_t v = (blockIdx.x + k * W) * kSliceTile + threadIdx.x;
+        const int i = k * kSliceTile + threadIdx.x;
         bool up = false;
-        if (x == EGS_TOP) {
-            hv = kHTop;
-        } else if (!member) {
-            hv = kHOut;
-        } else if (!small) {
-            members++;                               // decided by k_closure_big
-            hv = scale * ((long long)g.mg - x);
-        } else {
-            members++;
-            const long long h0 = scale * ((long long)g.mg - x);
-            if (owner1 ? !(fl & 1) : (fl & 2)) {
+        if (v < g.n) {
+            const int x = g.val[v];
+            const bool owner1 = v >= g.n0;
+            const int fl = lfl[i];
+            long long hv;
+            if (x == EGS_TOP) {
+                hv = kHTop;
+            } else if (!in_d_x(g, (int)v, x, B, last)) {
                 hv = kHOut;
-                up = true;
-                removed++;
-            } else if (best > h0) {
-                hv = best;
-                up = true;
-                if (best > bound) atomicOr(&flags[1], 1);
+            } else if (!(fl & 4)) {
+                members++;                               // above big_deg: decided by k_closure_big
+                hv = scale * ((long long)g.mg - x);
             } else {
-                hv = h0;
+                members++;
+                const long long h0 = scale * ((long long)g.mg - x);
+                const long long best = lbest[i];
+                if (owner1 ? !(fl & 1) : (fl & 2)) {
+                    hv = kHOut;
+                    up = true;
+                    removed++;
+                } else if (best > h0) {
+                    hv = best;
+                    up = true;
+                    if (best > bound) atomicOr(&flags[1], 1);
+                } else {
+                    hv = h0;
+                }
             }
+            h[v] = hv;
         }
-        h[v] = hv;
         changed += up;
+        // a wave covers 64 aligned vertices (two bitmap words); words wholly past n stay as they are
         const unsigned long long bal = __ballot(up);
-        if (lane == 0 || lane == 32) {
+        if ((lane == 0 || lane == 32) && v < g.n) {
             const int64_t word = v >> 5;
             const unsigned bw = (unsigned)(bal >> lane);
             bits[word] = bw;
@@ -300,7 +330,6 @@ __global__ __launch_bounds__(256) void k_closure_phase(DevGame g, int B, int las
             clear[word] = 0u;
         }
     }
-    if (!final_phase) return;
     for (int64_t i = bm_words(g.n) + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < bm_total(g.n);
          i += (int64_t)gridDim.x * blockDim.x)
         clear[i] = 0u;
@@ -759,14 +788,13 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
         if (hipMemsetAsync(cb.bfcnt, 0, sizeof(int) * kCntSlots, st) != hipSuccess) return -1;
         if (hipMemsetAsync(bm[0] + words, 0, sizeof(unsigned) * (bm_len - words), st) != hipSuccess) return -1;
         if (g.phases > 1) {
-            for (int s = 0; s < g.phases; s++) {
-                if (g.sshift)
-                    hipLaunchKernelGGL(k_closure_phase<true>, dim3(grid), dim3(256), 0, st, g, B, last, bound, s, cb.h,
-                                       bm[0], bm[1], cb.bfcnt, flags);
-                else
-                    hipLaunchKernelGGL(k_closure_phase<false>, dim3(grid), dim3(256), 0, st, g, B, last, bound, s, cb.h,
-                                       bm[0], bm[1], cb.bfcnt, flags);
-            }
+            const size_t lds = (size_t)g.phase_tiles * kSliceTile * (sizeof(long long) + sizeof(int));
+            if (g.sshift)
+                hipLaunchKernelGGL(k_closure_fused<true>, dim3(g.phase_grid), dim3(256), lds, st, g, B, last, bound,
+                                   cb.h, bm[0], bm[1], cb.bfcnt, flags);
+            else
+                hipLaunchKernelGGL(k_closure_fused<false>, dim3(g.phase_grid), dim3(256), lds, st, g, B, last, bound,
+                                   cb.h, bm[0], bm[1], cb.bfcnt, flags);
         } else {
             hipLaunchKernelGGL(k_closure, dim3(grid), dim3(256), 0, st, g, B, last, bound, cb.h, bm[0], bm[1], cb.bfcnt,
                                flags);

@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 
 #include "../../include/egs.h"
 #include "egs_device.h"
@@ -45,8 +46,6 @@ int check_gfx950(int device) {
     return EGS_OK;
 }
 
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
 // Loop bound of src/egsolver.c:64-90: a negative request means n*m*MG+1, saturated to
 // LONG_MAX-1 on overflow.
 int64_t egs_max_loop_c(int64_t m, int64_t n, int64_t mg, int64_t maxloop) {
@@ -76,10 +75,83 @@ constexpr int kCertCandidates = 8;    // gap thresholds tried per attempt (plus 
 constexpr int kCertEvery = 8;         // default rounds before the first proof attempt
 constexpr int kStopBatch = 4;         // rounds queued ahead of the host when a stop flag is polled
 
+double egs_now_ms() {
+    return 1e-6 * (double)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                      std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 int pow2ceil(int x) {
     int p = 1;
     while (p < x) p <<= 1;
     return p;
+}
+
+// Byte offsets of every array in a solver's device workspace (one block), for games up to
+// n_cap vertices and m_cap edges.  constexpr so that the size of the no-allocation
+// workspace below is known at compile time.
+struct Layout {
+    size_t ptr, edges, cptr, cpred, val, stamp, l0, l1, dbig, dhuge, sbig, shuge, lcnt, mode, ring, rounds, misc,
+        tile, fill, stats, statsum, sigma, h, hist, bfcnt, bits, hacc, hcnt, hbest, hflag, slen, sbase, sedges,
+        bigmask, total;
+    int sub_cap;
+    bool phased;
+};
+
+constexpr size_t align256c(size_t x) { return (x + 255) & ~(size_t)255; }
+
+constexpr Layout make_layout(int32_t n_cap, int64_t m_cap) {
+    Layout L{};
+    const size_t n1 = (size_t)n_cap + 1, m = (size_t)m_cap;
+    const size_t tiles = (size_t)(n_cap + 4095) / 4096 + 1;
+    const int64_t sc = (2 * (int64_t)n_cap + egs::kSub - 1) / egs::kSub;
+    L.sub_cap = sc > 256 ? (int)sc : 256;
+    const size_t list_ints = (size_t)egs::kSub * (size_t)L.sub_cap + n1;
+    size_t off = 0;
+    auto take = [&off](size_t bytes) {
+        const size_t o = off;
+        off += align256c(bytes);
+        return o;
+    };
+    L.ptr = take(4 * n1);
+    L.edges = take(8 * m);
+    L.cptr = take(4 * n1);
+    L.cpred = take(4 * m);
+    L.val = take(4 * n1);
+    L.stamp = take(4 * n1);
+    L.l0 = take(4 * list_ints);
+    L.l1 = take(4 * list_ints);
+    L.dbig = take(4 * n1);
+    L.dhuge = take(4 * n1);
+    L.sbig = take(4 * n1);
+    L.shuge = take(4 * n1);
+    L.lcnt = take(4 * 3 * egs::kListCounters);
+    L.mode = take(4 * 4);
+    L.ring = take(4 * 16);
+    L.rounds = take(4 * 16);
+    L.misc = take(8 * 16);
+    L.tile = take(4 * tiles);
+    L.fill = take(4 * n1);
+    L.stats = take(8 * 4 * (size_t)egs::kStatSlots);
+    L.statsum = take(8 * 8);
+    L.sigma = take(4 * n1);
+    L.h = take(8 * n1);
+    L.hist = take(4 * 256);
+    L.bfcnt = take(4 * 64);
+    L.bits = take(4 * 3 * (n1 / 32 + n1 / 1024 + 4));
+    L.hacc = take(4 * n1);
+    L.hcnt = take(4 * n1);
+    L.hbest = take(8 * n1);
+    L.hflag = take(4 * n1);
+    // phased dense rounds (large games): per tile and slice the vertices' edge counts (a
+    // byte each) and the wave bases, and the edges in (tile, slice, vertex) order
+    L.phased = n_cap >= egs::kPhaseMinN;
+    const size_t ptiles = (size_t)n_cap / egs::kSliceTile + 1;
+    L.slen = L.phased ? take(ptiles * egs::kMaxPhases * egs::kSliceTile) : 0;
+    L.sbase = L.phased ? take(4 * ptiles * egs::kMaxPhases * egs::kSliceWaves) : 0;
+    L.sedges = L.phased ? take(8 * m) : 0;
+    L.bigmask = L.phased ? take(8 * ((size_t)n_cap / 64 + 2)) : 0;
+    L.total = off;
+    return L;
 }
 
 }  // namespace
@@ -103,7 +175,8 @@ struct egs_solver {
     int *d_hacc = nullptr, *d_hcnt = nullptr;   // shared lifts of the vertices above huge_deg
     long long *d_hbest = nullptr;               // shared proof passes of the same vertices
     int *d_hflag = nullptr;
-    int *d_sptr = nullptr, *d_acc = nullptr;    // phased dense rounds (large n_cap)
+    uint8_t *d_slen = nullptr;                  // phased dense rounds (large n_cap)
+    int *d_sbase = nullptr;
     int2 *d_sedges = nullptr;
     unsigned long long *d_bigmask = nullptr;
     long long *d_h = nullptr;
@@ -127,6 +200,72 @@ struct egs_solver {
     const volatile uint32_t *stop_flag = nullptr;
     egs_stats st{};
 };
+
+// Point the solver's arrays into a workspace laid out by make_layout.
+static void bind_layout(egs_solver *s, char *b, const Layout &L) {
+    s->sub_cap = L.sub_cap;
+    s->d_ptr = (int *)(b + L.ptr);
+    s->d_edges = (int2 *)(b + L.edges);
+    s->d_cptr = (int *)(b + L.cptr);
+    s->d_cpred = (int *)(b + L.cpred);
+    s->d_val = (int *)(b + L.val);
+    s->d_stamp = (int *)(b + L.stamp);
+    s->d_list[0] = (int *)(b + L.l0);
+    s->d_list[1] = (int *)(b + L.l1);
+    s->d_dbig = (int *)(b + L.dbig);
+    s->d_dhuge = (int *)(b + L.dhuge);
+    s->d_sbig = (int *)(b + L.sbig);
+    s->d_shuge = (int *)(b + L.shuge);
+    s->d_lcnt = (int *)(b + L.lcnt);
+    s->d_mode = (int *)(b + L.mode);
+    s->d_ring = (int *)(b + L.ring);
+    s->d_rounds = (int *)(b + L.rounds);
+    s->d_misc = (int *)(b + L.misc);
+    s->d_tile = (int *)(b + L.tile);
+    s->d_fill = (int *)(b + L.fill);
+    s->d_stats = (unsigned long long *)(b + L.stats);
+    s->d_statsum = (unsigned long long *)(b + L.statsum);
+    s->d_sigma = (int *)(b + L.sigma);
+    s->d_h = (long long *)(b + L.h);
+    s->d_hist = (int *)(b + L.hist);
+    s->d_bfcnt = (int *)(b + L.bfcnt);
+    s->d_bits = (unsigned *)(b + L.bits);
+    s->d_hacc = (int *)(b + L.hacc);
+    s->d_hcnt = (int *)(b + L.hcnt);
+    s->d_hbest = (long long *)(b + L.hbest);
+    s->d_hflag = (int *)(b + L.hflag);
+    if (L.phased) {
+        s->d_slen = (uint8_t *)(b + L.slen);
+        s->d_sbase = (int *)(b + L.sbase);
+        s->d_sedges = (int2 *)(b + L.sedges);
+        s->d_bigmask = (unsigned long long *)(b + L.bigmask);
+    }
+}
+
+// Timing and probes use events when the solver owns them; a solver without events (the
+// no-allocation entry point) times on the host and waits on the whole stream instead.
+static hipError_t mark_start(egs_solver *s, double *host_t0) {
+    if (s->ev_a) return hipEventRecord(s->ev_a, s->stream);
+    *host_t0 = egs_now_ms();
+    return hipSuccess;
+}
+static hipError_t mark_stop(egs_solver *s, double host_t0, float *ms) {
+    if (s->ev_b) {
+        hipError_t e = hipEventRecord(s->ev_b, s->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(s->ev_b);
+        if (e == hipSuccess) e = hipEventElapsedTime(ms, s->ev_a, s->ev_b);
+        return e;
+    }
+    hipError_t e = hipStreamSynchronize(s->stream);
+    *ms = (float)(egs_now_ms() - host_t0);
+    return e;
+}
+static hipError_t probe_mark(egs_solver *s, int slot) {
+    return s->ev_probe[slot] ? hipEventRecord(s->ev_probe[slot], s->stream) : hipSuccess;
+}
+static hipError_t probe_wait(egs_solver *s, int slot) {
+    return s->ev_probe[slot] ? hipEventSynchronize(s->ev_probe[slot]) : hipStreamSynchronize(s->stream);
+}
 
 // Items in a list from its counters (sub-queues clamp at sub_cap; the rest overflowed).
 static int64_t list_total(const egs_solver *s, const int *c) {
@@ -221,70 +360,13 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
     if (hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) s->cus = 256;
     s->n_cap = n_cap;
     s->m_cap = m_cap;
-    const size_t n1 = (size_t)n_cap + 1;
-    const int tiles = (n_cap + 4095) / 4096 + 1;
-    s->sub_cap = std::max(256, (int)((2 * (int64_t)n_cap + egs::kSub - 1) / egs::kSub));
-    const size_t list_ints = (size_t)egs::kSub * s->sub_cap + n1;
-    size_t off = 0;
-    auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
-    size_t o_ptr = take(4 * n1), o_edges = take(8 * (size_t)m_cap), o_cptr = take(4 * n1),
-           o_cpred = take(4 * (size_t)m_cap), o_val = take(4 * n1), o_stamp = take(4 * n1),
-           o_l0 = take(4 * list_ints), o_l1 = take(4 * list_ints), o_dbig = take(4 * n1),
-           o_dhuge = take(4 * n1), o_sbig = take(4 * n1), o_shuge = take(4 * n1),
-           o_lcnt = take(4 * 3 * egs::kListCounters), o_mode = take(4 * 4),
-           o_ring = take(4 * 16), o_rounds = take(4 * 16), o_misc = take(8 * 16),
-           o_tile = take(4 * (size_t)tiles), o_fill = take(4 * n1),
-           o_stats = take(8 * 4 * (size_t)egs::kStatSlots), o_statsum = take(8 * 8),
-           o_sigma = take(4 * n1), o_h = take(8 * n1), o_hist = take(4 * 256), o_bfcnt = take(4 * 64),
-           o_bits = take(4 * 3 * (n1 / 32 + n1 / 1024 + 4)), o_hacc = take(4 * n1), o_hcnt = take(4 * n1),
-           o_hbest = take(8 * n1), o_hflag = take(4 * n1);
-    // phased dense rounds: slice offsets per tile, sliced edges, partial keys
-    const bool phased = n_cap >= egs::kPhaseMinN;
-    const size_t pn1 = ((size_t)n_cap / egs::kSliceTile + 1) * egs::kSliceStride;
-    const size_t o_sptr = phased ? take(4 * pn1) : 0, o_sedges = phased ? take(8 * (size_t)m_cap) : 0,
-                 o_acc = phased ? take(4 * n1) : 0, o_bigmask = phased ? take(8 * ((size_t)n_cap / 64 + 2)) : 0;
-    s->block_bytes = off;
-    if (hipMalloc(&s->block, off) != hipSuccess) {
+    const Layout L = make_layout(n_cap, m_cap);
+    s->block_bytes = L.total;
+    if (hipMalloc(&s->block, L.total) != hipSuccess) {
         egs_solver_destroy(s);
-        return fail(EGS_E_NOMEM, "hipMalloc of %zu bytes failed", off);
+        return fail(EGS_E_NOMEM, "hipMalloc of %zu bytes failed", L.total);
     }
-    char *b = (char *)s->block;
-    s->d_ptr = (int *)(b + o_ptr);
-    s->d_edges = (int2 *)(b + o_edges);
-    s->d_cptr = (int *)(b + o_cptr);
-    s->d_cpred = (int *)(b + o_cpred);
-    s->d_val = (int *)(b + o_val);
-    s->d_stamp = (int *)(b + o_stamp);
-    s->d_list[0] = (int *)(b + o_l0);
-    s->d_list[1] = (int *)(b + o_l1);
-    s->d_dbig = (int *)(b + o_dbig);
-    s->d_dhuge = (int *)(b + o_dhuge);
-    s->d_sbig = (int *)(b + o_sbig);
-    s->d_shuge = (int *)(b + o_shuge);
-    s->d_lcnt = (int *)(b + o_lcnt);
-    s->d_mode = (int *)(b + o_mode);
-    s->d_ring = (int *)(b + o_ring);
-    s->d_rounds = (int *)(b + o_rounds);
-    s->d_misc = (int *)(b + o_misc);
-    s->d_tile = (int *)(b + o_tile);
-    s->d_fill = (int *)(b + o_fill);
-    s->d_stats = (unsigned long long *)(b + o_stats);
-    s->d_statsum = (unsigned long long *)(b + o_statsum);
-    s->d_sigma = (int *)(b + o_sigma);
-    s->d_h = (long long *)(b + o_h);
-    s->d_hist = (int *)(b + o_hist);
-    s->d_bfcnt = (int *)(b + o_bfcnt);
-    s->d_bits = (unsigned *)(b + o_bits);
-    s->d_hacc = (int *)(b + o_hacc);
-    s->d_hcnt = (int *)(b + o_hcnt);
-    s->d_hbest = (long long *)(b + o_hbest);
-    s->d_hflag = (int *)(b + o_hflag);
-    if (phased) {
-        s->d_sptr = (int *)(b + o_sptr);
-        s->d_sedges = (int2 *)(b + o_sedges);
-        s->d_acc = (int *)(b + o_acc);
-        s->d_bigmask = (unsigned long long *)(b + o_bigmask);
-    }
+    bind_layout(s, (char *)s->block, L);
     if (hipHostMalloc((void **)&s->h_probe, 8192, hipHostMallocDefault) != hipSuccess) {
         egs_solver_destroy(s);
         return fail(EGS_E_NOMEM, "hipHostMalloc failed");
@@ -438,26 +520,42 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     s->cert_every = o->certify_every == 0 ? kCertEvery : (o->certify_every < 0 ? 0 : o->certify_every);
     s->cert_max_passes = o->cert_max_passes > 0 ? o->cert_max_passes : 1024;
     g.rise = s->cert_every > 0 ? s->d_sigma : nullptr;    // rise stamps only when proofs run
-    // phased dense rounds where the value array outgrows the L2 (and the buffers exist)
-    const bool want = o->slices == 0 ? g.n >= egs::kPhaseMinN : o->slices > 1;
-    g.phases = (want && s->d_sptr) ? egs::kPhases : 1;
-    g.slice_len = (g.n + egs::kPhases - 1) / egs::kPhases;
-    g.sptr = s->d_sptr;
+    // phased dense rounds where the value array outgrows the L2 (and the buffers exist):
+    // slices of 2^slice_shift successors, at most kMaxPhases of them
+    int want = o->slices == 0 ? (g.n >= egs::kPhaseMinN ? egs::kDefaultPhases : 1) : o->slices;
+    if (want < 1 || want > egs::kMaxPhases)
+        return fail(EGS_E_ARG, "slices must be 0 (auto) or 1..%d (got %d)", egs::kMaxPhases, o->slices);
+    g.phases = 1;
+    g.slice_shift = 31;
+    if (want > 1 && s->d_slen) {
+        int shift = 0;
+        while (((int64_t)(want) << shift) < g.n) shift++;          // want slices of 2^shift cover n
+        g.slice_shift = shift;
+        g.phases = (int)(((int64_t)g.n + (1LL << shift) - 1) >> shift);
+        if (g.big_deg > egs::kMaxSliceDeg) g.big_deg = egs::kMaxSliceDeg;   // lengths are bytes
+        if (g.huge_deg < g.big_deg) g.huge_deg = g.big_deg;
+        sh.use_wave = s->max_outdeg > g.big_deg;
+        sh.use_block = s->max_outdeg > g.huge_deg;
+    }
+    g.slen = s->d_slen;
+    g.sbase = s->d_sbase;
     g.sedges = s->d_sedges;
-    g.sshift = o->slices == 1 ? 0 : egs::pack_shift(g.n, s->wmax);   // 32-bit slice edges when they fit
-    g.acc = s->d_acc;
+    g.sshift = g.phases > 1 ? egs::pack_shift(g.n, s->wmax) : 0;   // 32-bit slice edges when they fit
     g.hacc = s->d_hacc;
     g.hcnt = s->d_hcnt;
     g.hbest = s->d_hbest;
     g.hflag = s->d_hflag;
     g.bigmask = s->d_bigmask;
-    // one lane per vertex up to two edges per slice on average; eight 256-thread workgroups
-    // per CU (a sweep on the 2^22-vertex game: 269 us per round against 282 with two lanes
-    // and 16 per CU, 352 with 9 per CU; 253 us since a lane keeps two edges in flight)
-    int pg = 1;
-    while (pg < 8 && pg * egs::kPhases * 2 < avg) pg <<= 1;
-    sh.phase_group = pg;
-    sh.phase_grid = egs::grid_for((int64_t)g.n * pg, 256, 8 * std::max(1, s->cus));
+    // The phased kernels run about as many workgroups as the device holds at once (eight of
+    // 256 threads per CU), each over a fixed set of tiles whose partial results sit in LDS.
+    const int64_t tiles = ((int64_t)g.n + egs::kSliceTile - 1) / egs::kSliceTile;
+    g.phase_grid = (int)std::min<int64_t>(tiles, 8 * (int64_t)std::max(1, s->cus));
+    g.phase_tiles = (int)((tiles + g.phase_grid - 1) / g.phase_grid);
+    if (g.phases > 1 && g.phase_tiles * egs::kSliceTile * 12 > 96 * 1024) {
+        // keep the closure kernel's partial state (12 bytes a vertex) within 96 KB of LDS
+        g.phase_tiles = 96 * 1024 / (egs::kSliceTile * 12);
+        g.phase_grid = (int)((tiles + g.phase_tiles - 1) / g.phase_tiles);
+    }
     return EGS_OK;
 }
 
@@ -475,7 +573,8 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     HIP_TRY(hipSetDevice(s->device));
     int rc = pick_shape(s, opt);
     if (rc) return rc;
-    HIP_TRY(hipEventRecord(s->ev_a, s->stream));
+    double t0 = 0;
+    HIP_TRY(mark_start(s, &t0));
     HIP_TRY(hipMemsetAsync(s->d_stats, 0, sizeof(unsigned long long) * 4 * egs::kStatSlots, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_rounds, 0, sizeof(int), s->stream));
     // The transpose only feeds the sparse round modes, and many solves never need them
@@ -494,12 +593,11 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     HIP_TRY(egs::launch_slices(s->g, s->stream));
     HIP_TRY(egs::launch_init(s->g, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_ring + 11, 0, sizeof(int), s->stream));
-    HIP_TRY(hipEventRecord(s->ev_b, s->stream));
+    float ms = 0.f;
+    HIP_TRY(mark_stop(s, t0, &ms));
     HIP_TRY(hipMemcpyAsync(s->h_probe + kPinInit, s->d_lcnt + 1 * egs::kListCounters,
                            sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, s->ev_a, s->ev_b));
     memset(&s->st, 0, sizeof(s->st));
     s->st.ms_prepare = ms;
     s->st.initial_active = (int32_t)list_total(s, s->h_probe + kPinInit);
@@ -535,7 +633,8 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
     int64_t limit = s->phase_cap - (s->next_round - 1);
     if (max_phases >= 0 && max_phases < limit) limit = max_phases;
     if (s->next_round + limit > 0x7ffffff0LL) limit = 0x7ffffff0LL - s->next_round;
-    HIP_TRY(hipEventRecord(s->ev_a, s->stream));
+    double t0 = 0;
+    HIP_TRY(mark_start(s, &t0));
     int64_t launched = 0;
     int pending = -1;     // probe slot in flight
     bool done = false;
@@ -597,9 +696,9 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
         if (!s->csc_built)
             HIP_TRY(hipMemcpyAsync(s->h_probe + kPinProbe + slot * 128 + kProbeWant, s->d_ring + 13, sizeof(int),
                                    hipMemcpyDeviceToHost, s->stream));
-        HIP_TRY(hipEventRecord(s->ev_probe[slot], s->stream));
+        HIP_TRY(probe_mark(s, slot));
         if (pending >= 0) {
-            HIP_TRY(hipEventSynchronize(s->ev_probe[pending]));
+            HIP_TRY(probe_wait(s, pending));
             if (list_total(s, s->h_probe + kPinProbe + pending * 128) == 0) done = true;
             // enough rounds wanted to go sparse: build the transpose now
             if (!done && !s->csc_built && s->h_probe[kPinProbe + pending * 128 + kProbeWant] >= kSparseWantRounds)
@@ -609,11 +708,10 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
         if (s->stop_flag && *s->stop_flag) break;
         if (batch < s->batch_rounds) batch = std::min(s->batch_rounds, batch * 2);
     }
-    HIP_TRY(hipEventRecord(s->ev_b, s->stream));
+    float ms = 0.f;
+    HIP_TRY(mark_stop(s, t0, &ms));
     int rc = collect_stats(s);
     if (rc) return rc;
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, s->ev_a, s->ev_b));
     s->st.ms_iterate += ms;
     if (st) *st = s->st;
     return s->st.converged ? 1 : 0;

@@ -45,19 +45,24 @@ struct DevGame {
     int *rounds_done;     // last round that was not MODE_DONE
     int *rise;            // n: round in which each vertex last rose (TOP proof candidates);
                           // null when proofs are off
-    // Dense rounds in successor-range phases (phases > 1): phase s relaxes the edges whose
-    // successor lies in [s * slice_len, (s+1) * slice_len), so its value gathers stay in L2.
+    // Dense rounds in successor-range phases (phases > 1): in phase s a vertex folds its
+    // edges whose successor lies in slice s = [s << slice_shift, (s + 1) << slice_shift), so
+    // the value gathers of the whole grid come from one slice small enough to stay in L2.
     int phases;           // 1 = one in-place sweep (body_dense)
-    int slice_len;
-    // Tiles of kSliceTile vertices: tile T's low-degree edges lie in its own CSR range
-    // from ptr[T * kSliceTile], grouped by (slice, vertex); its kSliceStride offsets
-    // sptr[T * kSliceStride + s * kSliceTile + j] start (slice s, vertex T*kSliceTile + j)
-    // and the last one ends the tile, so segment i ends where i + 1 starts.
-    const int *sptr;
+    int slice_shift;
+    // Slice layout in tiles of kSliceTile vertices: tile T's low-degree edges lie in its own
+    // CSR range from ptr[T * kSliceTile], grouped by (slice, vertex).  For tile T and slice
+    // s, slen[(T * phases + s) * kSliceTile + j] is the number of edges of vertex
+    // T * kSliceTile + j into slice s (0 above big_deg and past n) and
+    // sbase[(T * phases + s) * kSliceWaves + w] is where the edges of wave w's 64 vertices
+    // start; a vertex's edges follow those of the lower lanes of its wave.
+    const uint8_t *slen;
+    const int *sbase;
     const void *sedges;   // uint32 (w << sshift) | succ when sshift > 0, else int2 {succ, w}
     int sshift;
     const unsigned long long *bigmask;   // bit v: vertex v is above big_deg (lifted by the wave/block kernels)
-    int *acc;             // n partial keys carried between phases (and the closure's flags)
+    int phase_grid;       // workgroups of the phased kernels (about what the device holds at once)
+    int phase_tiles;      // tiles of the slice layout per workgroup, at most
     int *hacc, *hcnt;     // n each: partial keys and finished chunks of the vertices above
                           // huge_deg (TOP and 0 between rounds)
     long long *hbest;     // n: partial longest-path candidates of the vertices above huge_deg
@@ -65,15 +70,12 @@ struct DevGame {
     int *hflag;           // n: bit 0 a successor changed, bit 1 an edge leaves D (0 between passes)
 };
 
-constexpr int kPhases = 4;                 // successor slices of a phased dense round
+constexpr int kMaxPhases = 16;             // successor slices of a phased dense round, at most
+constexpr int kDefaultPhases = 8;          // slices of 2 MB of values on the 2^22-vertex game
 constexpr int kPhaseMinN = 1 << 20;        // below this the value array fits the L2 anyway
 constexpr int kSliceTile = 256;            // vertices per tile of the slice layout
-constexpr int kSliceStride = kPhases * kSliceTile + 1;
-
-// Index in sptr of the segment (slice s, vertex v); the segment ends at the next index.
-__device__ __forceinline__ size_t slice_seg(int64_t v, int s) {
-    return (size_t)(v / kSliceTile) * kSliceStride + (size_t)s * kSliceTile + (size_t)(v % kSliceTile);
-}
+constexpr int kSliceWaves = kSliceTile / 64;
+constexpr int kMaxSliceDeg = 255;          // a vertex's edges into one slice fit a byte
 
 // Edge k of the slice layout: successor and weight.
 template <bool PACK>
@@ -95,8 +97,6 @@ struct LaunchShape {
     int grid;        // workgroups of k_round
     int use_wave, use_block;
     int wave_grid, block_grid;
-    int phase_group; // lanes per vertex in k_phase (phased dense rounds)
-    int phase_grid;
 };
 
 hipError_t launch_pack(const int *succ, const int *w, int2 *edges, int64_t m, hipStream_t st);

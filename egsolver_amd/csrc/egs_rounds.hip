@@ -473,85 +473,95 @@ __global__ __launch_bounds__(256) void k_round(DevGame g, int r) {
     flush_stats(g, acc);
 }
 
-// One phase s of a phased dense round.  Every low-degree vertex folds the keys of its
-// edges into slice s into acc[v]; the last phase applies the lift: old values are read by
-// every phase (the round is Jacobi within itself), the last writes the risen ones, stamps
-// them and stages them for the next round like body_dense.  Vertices above big_deg have
-// no edges in the slices: the static wave/block kernels lift them after the phases.
-constexpr int kPhaseU = 2;   // edges in flight per lane in k_phase (4: same rounds, slower solves)
+// Exclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ int wave_excl_scan(int x) {
+    const int lane = threadIdx.x & 63;
+    int inc = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += y;
+    }
+    return inc - x;
+}
 
-template <int G, bool PACK>
-__global__ __launch_bounds__(256) void k_phase(DevGame g, int r, int s) {
+// A phased dense round in one launch.  Workgroup b takes the tiles b, b + W, b + 2W, ...
+// (W = the grid) and walks the phases s = 0 .. phases-1 in order, folding each vertex's
+// edges into slice s into its partial key, kept in LDS for the whole round (no per-vertex
+// array between phases).  The grid is about what the device holds at once, so its
+// workgroups move through the slices together and the value gathers of the moment come
+// from one slice, which stays in L2.  After the last phase each low-degree vertex applies
+// its lift, stamps its rise and is staged for the next round like body_dense.  Values are
+// read in place: a workgroup still in an early phase may see values another has already
+// raised in this round, which only helps (values never pass the least fixpoint, and every
+// vertex is swept again next round while anything rose).  Vertices above big_deg have no
+// edges in the slices; the static wave/block kernels lift them after this kernel.
+constexpr int kPhaseU = 2;   // edges in flight per lane
+
+template <bool PACK>
+__global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int r) {
+    extern __shared__ int keys[];                 // [tile of this workgroup][kSliceTile]
     __shared__ int sbuf[kLdsCap];
     __shared__ int scnt;
     if (g.mode[r % 3] != MODE_DENSE) return;
-    const bool last = s == g.phases - 1;
     if (threadIdx.x == 0) scnt = 0;
     __syncthreads();
     const Stage st{sbuf, &scnt, g.list[(r + 1) & 1], g.lcnt + ((r + 1) % 3) * kListCounters,
                    (int)(blockIdx.x % kSub)};
     LiftAcc acc;
-    constexpr int ITEMS = 256 / G;
-    const int gl = threadIdx.x % G;
-    int it = 0;
-    for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < g.n; t0 += (int64_t)gridDim.x * ITEMS, it++) {
-        const int v = (int)(t0 + threadIdx.x / G);
-        const bool valid = v < g.n;
-        int b = 0, e = 0;
-        if (valid) {
-            const size_t i = slice_seg(v, s);
-            b = g.sptr[i];
-            e = g.sptr[i + 1];
-        }
-        const bool owner1 = v >= g.n0;
-        int key = EGS_TOP;
-        // kPhaseU edges per lane at a time: their loads, then their gathers, are all in
-        // flight together (an edge past e re-reads edge k0, a valid index, and is masked)
-        for (int k0 = b + gl; k0 < e; k0 += G * kPhaseU) {
-            int u[kPhaseU], w[kPhaseU], x[kPhaseU];
+    const int P = g.phases;
+    const int64_t tiles = ((int64_t)g.n + kSliceTile - 1) / kSliceTile;
+    const int64_t W = gridDim.x;
+    const int K = blockIdx.x < tiles ? (int)((tiles - blockIdx.x + W - 1) / W) : 0;
+    const int wave = threadIdx.x >> 6;
+    uint64_t edges = 0;
+    for (int s = 0; s < P; s++) {
+        for (int k = 0; k < K; k++) {
+            const int64_t T = blockIdx.x + k * W;
+            const size_t seg = (size_t)T * P + s;
+            const int len = g.slen[seg * kSliceTile + threadIdx.x];
+            const int b = g.sbase[seg * kSliceWaves + wave] + wave_excl_scan(len);
+            const int e = b + len;
+            const bool owner1 = T * kSliceTile + threadIdx.x >= g.n0;
+            int key = EGS_TOP;
+            // kPhaseU edges per lane at a time: their loads, then their gathers, are all in
+            // flight together (a slot past e re-reads edge k0, a valid index, and is masked)
+            for (int k0 = b; k0 < e; k0 += kPhaseU) {
+                int u[kPhaseU], w[kPhaseU], x[kPhaseU];
 #pragma unroll
-            for (int j = 0; j < kPhaseU; j++) slice_edge<PACK>(g, k0 + j * G < e ? k0 + j * G : k0, u[j], w[j]);
+                for (int j = 0; j < kPhaseU; j++) slice_edge<PACK>(g, k0 + j < e ? k0 + j : k0, u[j], w[j]);
 #pragma unroll
-            for (int j = 0; j < kPhaseU; j++) x[j] = k0 + j * G < e ? g.val[u[j]] : EGS_TOP;
+                for (int j = 0; j < kPhaseU; j++) x[j] = k0 + j < e ? g.val[u[j]] : EGS_TOP;
 #pragma unroll
-            for (int j = 0; j < kPhaseU; j++)
-                if (k0 + j * G < e) key = min(key, egs_key(egs_ominus(x[j], w[j], g.mg), owner1));
-        }
-        key = egs_group_min<G>(key);
-        bool lifted = false;
-        if (valid && gl == 0) {
-            acc.edges += (uint32_t)(e - b);
-            if (!last) {
-                if (s == 0) g.acc[v] = key;
-                else if (key != EGS_TOP && key < g.acc[v]) g.acc[v] = key;
-            } else if (!((g.bigmask[v >> 6] >> (v & 63)) & 1ull)) {
-                const int a = g.phases > 1 ? min(g.acc[v], key) : key;
-                const int t = egs_unkey(a, owner1);
-                const int old = g.val[v];
-                acc.processed++;
-                if (old < t) {
-                    g.val[v] = t;
-                    if (g.rise) g.rise[v] = r;
-                    acc.lifted++;
-                    lifted = true;
-                }
+                for (int j = 0; j < kPhaseU; j++)
+                    if (k0 + j < e) key = min(key, egs_key(egs_ominus(x[j], w[j], g.mg), owner1));
             }
-        }
-        if (last) {
-            stage_push(g, st, lifted, v);
-            if ((it & 7) == 7) stage_flush(g, st, kLdsCap / 2);
+            edges += (uint32_t)len;
+            int &slot = keys[k * kSliceTile + threadIdx.x];
+            slot = s == 0 ? key : min(slot, key);
         }
     }
-    if (last) stage_flush(g, st, 1);
+    for (int k = 0; k < K; k++) {
+        const int64_t T = blockIdx.x + k * W;
+        const int v = (int)(T * kSliceTile + threadIdx.x);
+        bool lifted = false;
+        if (v < g.n && !((g.bigmask[v >> 6] >> (v & 63)) & 1ull)) {
+            const int t = egs_unkey(keys[k * kSliceTile + threadIdx.x], v >= g.n0);
+            const int old = g.val[v];
+            acc.processed++;
+            if (old < t) {
+                g.val[v] = t;
+                if (g.rise) g.rise[v] = r;
+                acc.lifted++;
+                lifted = true;
+            }
+        }
+        stage_push(g, st, lifted, v);
+        if ((k & 7) == 7) stage_flush(g, st, kLdsCap / 2);
+    }
+    stage_flush(g, st, 1);
+    acc.edges = edges;
     flush_stats(g, acc);
-}
-
-// Slice of a successor: phases of slice_len successors each (comparisons, no division).
-__device__ __forceinline__ int slice_of(int u, int slice_len) {
-    int q = 0;
-#pragma unroll
-    for (int b = 1; b < kPhases; b++) q += u >= b * slice_len;
-    return q;
 }
 
 // Edges read ahead per thread in the slice set-up kernel (one thread per vertex).
@@ -559,16 +569,18 @@ constexpr int kSliceU = 8;
 
 // Slice layout of the low-degree vertices' edges in one pass, one tile of kSliceTile
 // vertices per workgroup iteration and no global scan: each thread counts its vertex's
-// edges per slice, the workgroup scans the tile's counts in (slice, vertex) order, writes
-// the tile's kSliceStride offsets from ptr[first vertex of the tile], and each thread then
-// places its edges (read again, mostly from L2).  Vertices past n and those above big_deg
-// get empty segments.  The same pass marks the vertices above big_deg in bigmask (a wave
-// ballot) and decides k_init's initial activity of the others into stamp.
-__global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, int *__restrict__ sptr,
-                                                           void *__restrict__ sedges,
+// edges per slice (the lengths), the workgroup sums them per (slice, wave) and scans the
+// sums in that order from ptr of the tile's first vertex (the wave bases), and each thread
+// then places its edges (read again, mostly from L2) after those of the lower lanes of its
+// wave.  Vertices past n and those above big_deg get empty segments.  The same pass marks
+// the vertices above big_deg in bigmask (a wave ballot) and decides k_init's initial
+// activity of the others into stamp.
+__global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *__restrict__ slen,
+                                                           int *__restrict__ sbase, void *__restrict__ sedges,
                                                            unsigned long long *__restrict__ bigmask) {
-    __shared__ int s_off[kPhases * kSliceTile];
-    __shared__ int s_wave[kSliceTile / 64];
+    __shared__ int s_wt[kMaxPhases * kSliceWaves];
+    const int P = g.phases;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t tiles = ((int64_t)g.n + kSliceTile - 1) / kSliceTile;
     for (int64_t T = blockIdx.x; T < tiles; T += gridDim.x) {
         const int64_t first = T * kSliceTile;
@@ -581,7 +593,7 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, int *__re
         }
         const bool big = valid && e - s > g.big_deg;
         const bool mine = valid && !big;
-        int c[kPhases] = {};
+        int c[kMaxPhases] = {};
         if (mine) {
             bool all_neg = true, any_neg = false;
             for (int k0 = s; k0 < e; k0 += kSliceU) {
@@ -591,9 +603,9 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, int *__re
 #pragma unroll
                 for (int j = 0; j < kSliceU; j++) {
                     if (ew[j].x < 0) break;
-                    const int sl = slice_of(ew[j].x, g.slice_len);
+                    const int sl = ew[j].x >> g.slice_shift;
 #pragma unroll
-                    for (int q = 0; q < kPhases; q++) c[q] += sl == q;
+                    for (int q = 0; q < kMaxPhases; q++) c[q] += sl == q;
                     all_neg &= ew[j].y < 0;
                     any_neg |= ew[j].y < 0;
                 }
@@ -602,31 +614,33 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, int *__re
             g.stamp[v] = ((v < g.n0) ? all_neg : any_neg) ? 1 : 0;
         }
         const unsigned long long word = __ballot(big);
-        if ((threadIdx.x & 63) == 0 && valid) bigmask[v >> 6] = word;
+        if (lane == 0 && valid) bigmask[v >> 6] = word;
+        uint8_t *len_out = slen + (size_t)T * P * kSliceTile;
 #pragma unroll
-        for (int q = 0; q < kPhases; q++) s_off[q * kSliceTile + threadIdx.x] = c[q];
-        __syncthreads();
-        // exclusive scan in (slice, vertex) order; thread t takes items t*kPhases .. +kPhases-1
-        int loc[kPhases], run = 0;
-#pragma unroll
-        for (int k = 0; k < kPhases; k++) {
-            loc[k] = run;
-            run += s_off[threadIdx.x * kPhases + k];
+        for (int q = 0; q < kMaxPhases; q++) {
+            if (q < P) {
+                len_out[q * kSliceTile + threadIdx.x] = (uint8_t)c[q];
+                int t = c[q];
+                for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
+                if (lane == 0) s_wt[q * kSliceWaves + wave] = t;
+            }
         }
-        int total;
-        const int ex = egs_block_exclusive_scan(run, s_wave, total);
-#pragma unroll
-        for (int k = 0; k < kPhases; k++) s_off[threadIdx.x * kPhases + k] = ex + loc[k];
         __syncthreads();
-        const int base = g.ptr[first];
-        int *out = sptr + T * kSliceStride;
+        // (slice, wave) sums scanned in that order: at most kMaxPhases * kSliceWaves = 64 items
+        if (threadIdx.x < 64) {
+            const int x = threadIdx.x < P * kSliceWaves ? s_wt[threadIdx.x] : 0;
+            const int ex = wave_excl_scan(x);
+            if (threadIdx.x < P * kSliceWaves) {
+                const int b = g.ptr[first] + ex;
+                s_wt[threadIdx.x] = b;
+                sbase[(size_t)T * P * kSliceWaves + threadIdx.x] = b;
+            }
+        }
+        __syncthreads();
+        int pos[kMaxPhases];
 #pragma unroll
-        for (int q = 0; q < kPhases; q++) out[q * kSliceTile + threadIdx.x] = base + s_off[q * kSliceTile + threadIdx.x];
-        if (threadIdx.x == 0) out[kPhases * kSliceTile] = base + total;
+        for (int q = 0; q < kMaxPhases; q++) pos[q] = q < P ? s_wt[q * kSliceWaves + wave] + wave_excl_scan(c[q]) : 0;
         if (mine) {
-            int pos[kPhases];
-#pragma unroll
-            for (int q = 0; q < kPhases; q++) pos[q] = base + s_off[q * kSliceTile + threadIdx.x];
             for (int k0 = s; k0 < e; k0 += kSliceU) {
                 int2 ew[kSliceU];
 #pragma unroll
@@ -634,9 +648,9 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, int *__re
 #pragma unroll
                 for (int j = 0; j < kSliceU; j++) {
                     if (ew[j].x < 0) break;
-                    const int sl = slice_of(ew[j].x, g.slice_len);
+                    const int sl = ew[j].x >> g.slice_shift;
 #pragma unroll
-                    for (int q = 0; q < kPhases; q++) {
+                    for (int q = 0; q < kMaxPhases; q++) {
                         if (sl != q) continue;
                         if (g.sshift) ((uint32_t *)sedges)[pos[q]++] = ((uint32_t)ew[j].y << g.sshift) | (uint32_t)ew[j].x;
                         else ((int2 *)sedges)[pos[q]++] = ew[j];
@@ -644,7 +658,7 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, int *__re
                 }
             }
         }
-        __syncthreads();    // s_off is rewritten for the next tile
+        __syncthreads();    // s_wt is rewritten for the next tile
     }
 }
 
@@ -790,24 +804,11 @@ hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int r, hipStrea
     default: hipLaunchKernelGGL(k_round<64>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
     }
     if (g.phases > 1) {
-        for (int q = 0; q < g.phases; q++) {
-            const dim3 gr(sh.phase_grid), bl(256);
-            if (g.sshift) {
-                switch (sh.phase_group) {
-                case 1: hipLaunchKernelGGL((k_phase<1, true>), gr, bl, 0, st, g, r, q); break;
-                case 2: hipLaunchKernelGGL((k_phase<2, true>), gr, bl, 0, st, g, r, q); break;
-                case 4: hipLaunchKernelGGL((k_phase<4, true>), gr, bl, 0, st, g, r, q); break;
-                default: hipLaunchKernelGGL((k_phase<8, true>), gr, bl, 0, st, g, r, q); break;
-                }
-            } else {
-                switch (sh.phase_group) {
-                case 1: hipLaunchKernelGGL((k_phase<1, false>), gr, bl, 0, st, g, r, q); break;
-                case 2: hipLaunchKernelGGL((k_phase<2, false>), gr, bl, 0, st, g, r, q); break;
-                case 4: hipLaunchKernelGGL((k_phase<4, false>), gr, bl, 0, st, g, r, q); break;
-                default: hipLaunchKernelGGL((k_phase<8, false>), gr, bl, 0, st, g, r, q); break;
-                }
-            }
-        }
+        const size_t lds = (size_t)g.phase_tiles * kSliceTile * sizeof(int);
+        if (g.sshift)
+            hipLaunchKernelGGL(k_dense_fused<true>, dim3(g.phase_grid), dim3(256), lds, st, g, r);
+        else
+            hipLaunchKernelGGL(k_dense_fused<false>, dim3(g.phase_grid), dim3(256), lds, st, g, r);
     }
     if (sh.use_wave) hipLaunchKernelGGL(k_lift_wave, dim3(sh.wave_grid), dim3(256), 0, st, g, r);
     if (sh.use_block) hipLaunchKernelGGL(k_lift_block, dim3(sh.block_grid), dim3(256), 0, st, g, r);
@@ -817,7 +818,7 @@ hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int r, hipStrea
 hipError_t launch_slices(const DevGame &g, hipStream_t st) {
     if (g.phases <= 1) return hipSuccess;
     hipLaunchKernelGGL(k_slice_build, dim3(grid_for(g.n, kSliceTile, 4096)), dim3(kSliceTile), 0, st, g,
-                       const_cast<int *>(g.sptr), const_cast<void *>(g.sedges),
+                       const_cast<uint8_t *>(g.slen), const_cast<int *>(g.sbase), const_cast<void *>(g.sedges),
                        const_cast<unsigned long long *>(g.bigmask));
     return hipGetLastError();
 }

@@ -157,7 +157,8 @@ def test_phased_dense_rounds_vs_oracle(oracle, seed):
     with E.Solver(1 << 20, max(g.m, 1 << 22)) as s:
         s.load(g)
         for opt in (E.options(slices=4), E.options(slices=4, certify_every=-1), E.options(slices=4, dense_pct=1),
-                    E.options(slices=4, group=1, big_deg=4)):
+                    E.options(slices=4, group=1, big_deg=4), E.options(slices=16), E.options(slices=3, certify_every=2),
+                    E.options(slices=8, certify_every=2, big_deg=300)):
             s.prepare(opt)
             conv, st = s.iterate()
             assert conv
