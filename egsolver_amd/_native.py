@@ -165,6 +165,8 @@ EGS_SIGNATURES = {
     "egs_solver_count_violations": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "egs_solver_set_values": (ctypes.c_int, [_vp, _vp]),
     "egs_solver_top_certificate": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, ctypes.POINTER(ctypes.c_int32)]),
+    "egs_solve_noalloc": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _vp, _vp,
+                                         _vp, _vp, ctypes.POINTER(EgsOptions), ctypes.POINTER(EgsStats), _vp]),
     "egs_host_alloc": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_size_t]),
     "egs_host_free": (ctypes.c_int, [_vp]),
     "egs_runtime_error": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),

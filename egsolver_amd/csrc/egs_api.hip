@@ -806,4 +806,81 @@ int egs_solve(int device, int32_t n, int32_t n0, int64_t m, int32_t mg, const in
     return rc;
 }
 
+// ---------------------------------------------------------------------------------------
+// Solve without allocating: for callers that cannot allocate device memory (code under HIP
+// graph capture, tight serving loops, harnesses that forbid hipMalloc).  The workspace of
+// the one-game solver for games up to kNoAllocN vertices and kNoAllocM edges is a
+// file-scope device array; each device that runs it holds its own copy until the process
+// ends, found once per device with hipGetSymbolAddress (the array's name is not a device
+// address in host code).  The host side of the solver is a static object per device.
+// Calls that overlap on one device share both and race.
+// ---------------------------------------------------------------------------------------
+
+namespace {
+
+constexpr int32_t kNoAllocN = 1 << 16;
+constexpr int64_t kNoAllocM = 1 << 20;
+constexpr size_t kScratchBytes = make_layout(kNoAllocN, kNoAllocM).total;
+constexpr int kNoAllocDevices = 64;
+
+struct NoAllocSlot {
+    egs_solver s;
+    int probe[2048];     // the solver's read-back area (pageable; every read waits on the stream)
+    bool ready = false;
+};
+NoAllocSlot g_noalloc[kNoAllocDevices];
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static int noalloc_setup(int dev) {
+    NoAllocSlot &sl = g_noalloc[dev];
+    if (sl.ready) return EGS_OK;
+    int rc = check_gfx950(dev);
+    if (rc) return rc;
+    char *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(EGS_E_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+    egs_solver &s = sl.s;
+    s.device = dev;
+    if (hipDeviceGetAttribute(&s.cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) s.cus = 256;
+    s.n_cap = kNoAllocN;
+    s.m_cap = kNoAllocM;
+    s.block = nullptr;                 // not owned: never freed
+    s.block_bytes = kScratchBytes;
+    bind_layout(&s, ptr, make_layout(kNoAllocN, kNoAllocM));
+    s.h_probe = sl.probe;
+    s.h_probe[kPinOne] = 1;
+    sl.ready = true;
+    return EGS_OK;
+}
+
+int egs_solve_noalloc(int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *d_ptr, const int32_t *d_succ,
+                      const int32_t *d_w, int32_t *d_values, const egs_options *opt, egs_stats *st, void *stream) {
+    if (n < 1 || n0 < 0 || n0 > n || m < 1 || mg < 0 || !d_ptr || !d_succ || !d_w || !d_values)
+        return fail(EGS_E_ARG, "egs_solve_noalloc: bad game (n=%d n0=%d m=%lld mg=%d)", n, n0, (long long)m, mg);
+    if (n > kNoAllocN || m > kNoAllocM)
+        return fail(EGS_E_CAPACITY,
+                    "egs_solve_noalloc: game (n=%d, m=%lld) exceeds the fixed workspace (%d vertices, %lld edges)", n,
+                    (long long)m, kNoAllocN, (long long)kNoAllocM);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocDevices) return fail(EGS_E_ARG, "egs_solve_noalloc: device %d out of range", dev);
+    int rc = noalloc_setup(dev);
+    if (rc) return rc;
+    egs_solver *s = &g_noalloc[dev].s;
+    s->stream = (hipStream_t)stream;
+    s->stop_flag = nullptr;
+    rc = egs_solver_load(s, EGS_MEM_DEVICE, n, n0, m, mg, d_ptr, d_succ, d_w);
+    if (!rc) rc = egs_solver_prepare(s, opt);
+    if (rc) return rc;
+    rc = egs_solver_iterate(s, -1, st);
+    if (rc < 0) return rc;
+    HIP_TRY(hipMemcpyAsync(d_values, s->d_val, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return rc;
+}
+
 }  // extern "C"

@@ -186,6 +186,20 @@ int egs_solver_set_values(egs_solver *s, const int32_t *host_values);
  * the partial potentials), or an error code; *passes (may be NULL) gets the passes run. */
 int egs_solver_top_certificate(egs_solver *s, int32_t max_passes, int64_t *host_h, int32_t *passes);
 
+/* No reference counterpart.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): egs_solve for a
+ * game already on the device, with no allocation of any kind.  The solver's workspace is
+ * a fixed array in the library's device memory (18 MiB per device that runs it,
+ * held until the process ends), so games are limited to 65,536 vertices and 1,048,576
+ * edges (EGS_E_CAPACITY beyond).  d_ptr[n+1], d_succ[m], d_w[m] and d_values[n] are device
+ * arrays of the current device; work runs on `stream` (a hipStream_t; NULL = the null
+ * stream), which is synchronised before the call returns; host-side timing.  Every call on
+ * one device shares that device's workspace, which keeps the last call's contents: calls
+ * that overlap on one device race (serialise them).  Returns 1 at the fixpoint, 0 when
+ * stopped by the options' cap or stop flag, or an error code. */
+int egs_solve_noalloc(int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *d_ptr, const int32_t *d_succ,
+                      const int32_t *d_w, int32_t *d_values, const egs_options *opt, egs_stats *st, void *stream);
+
 /* ---- many games in one launch ---------------------------------------------------- */
 
 /* No reference counterpart (the reference solves one game per process).  Games are
