@@ -91,8 +91,9 @@ int pow2ceil(int x) {
 // workspace below is known at compile time.
 struct Layout {
     size_t ptr, edges, cptr, cpred, val, stamp, l0, l1, dbig, dhuge, sbig, shuge, lcnt, mode, ring, rounds, misc,
-        tile, fill, stats, statsum, sigma, h, hist, bfcnt, bits, hacc, hcnt, hbest, hflag, slen, sbase, sedges,
-        bigmask, total;
+        tile, fill, stats, statsum, sigma, h, hist, bfcnt, bits, hacc, hcnt, hbest, hflag, tcounts, ttot, tbstart,
+        titems, slen, sbase, sedges, bigmask, total;
+    int64_t tcounts_cap;
     int sub_cap;
     bool phased;
 };
@@ -142,6 +143,14 @@ constexpr Layout make_layout(int32_t n_cap, int64_t m_cap) {
     L.hcnt = take(4 * n1);
     L.hbest = take(8 * n1);
     L.hflag = take(4 * n1);
+    // sorted transpose: per (edge tile, destination bucket) counts, bucket sizes and starts,
+    // and the edges as (source, destination) items grouped by bucket
+    const int64_t nb_cap = std::min<int64_t>(4096, ((int64_t)n_cap + 2047) / 2048);
+    L.tcounts_cap = ((m_cap + egs::kTrTileEdges - 1) / egs::kTrTileEdges) * nb_cap;
+    L.tcounts = take(4 * (size_t)L.tcounts_cap);
+    L.ttot = take(4 * 4096);
+    L.tbstart = take(4 * 4097);
+    L.titems = take(8 * m);
     // phased dense rounds (large games): per tile and slice the vertices' edge counts (a
     // byte each) and the wave bases, and the edges in (tile, slice, vertex) order
     L.phased = n_cap >= egs::kPhaseMinN;
@@ -175,6 +184,9 @@ struct egs_solver {
     int *d_hacc = nullptr, *d_hcnt = nullptr;   // shared lifts of the vertices above huge_deg
     long long *d_hbest = nullptr;               // shared proof passes of the same vertices
     int *d_hflag = nullptr;
+    int *d_tcounts = nullptr, *d_ttot = nullptr, *d_tbstart = nullptr;   // sorted transpose
+    int2 *d_titems = nullptr;
+    int64_t tcounts_cap = 0;
     uint8_t *d_slen = nullptr;                  // phased dense rounds (large n_cap)
     int *d_sbase = nullptr;
     int2 *d_sedges = nullptr;
@@ -234,6 +246,11 @@ static void bind_layout(egs_solver *s, char *b, const Layout &L) {
     s->d_hcnt = (int *)(b + L.hcnt);
     s->d_hbest = (long long *)(b + L.hbest);
     s->d_hflag = (int *)(b + L.hflag);
+    s->d_tcounts = (int *)(b + L.tcounts);
+    s->d_ttot = (int *)(b + L.ttot);
+    s->d_tbstart = (int *)(b + L.tbstart);
+    s->d_titems = (int2 *)(b + L.titems);
+    s->tcounts_cap = L.tcounts_cap;
     if (L.phased) {
         s->d_slen = (uint8_t *)(b + L.slen);
         s->d_sbase = (int *)(b + L.sbase);
@@ -522,7 +539,10 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     g.rise = s->cert_every > 0 ? s->d_sigma : nullptr;    // rise stamps only when proofs run
     // phased dense rounds where the value array outgrows the L2 (and the buffers exist):
     // slices of 2^slice_shift successors, at most kMaxPhases of them
-    int want = o->slices == 0 ? (g.n >= egs::kPhaseMinN ? egs::kDefaultPhases : 1) : o->slices;
+    // auto: phases once the value array outgrows 4 MB (at 2^20 vertices one in-place sweep
+    // wins: the skewed 2^20 game solved in 13.6 ms in 81 rounds, against 15.8-16.3 ms in 98-99
+    // rounds with 2-8 phases)
+    int want = o->slices == 0 ? (g.n > egs::kPhaseMinN ? egs::kDefaultPhases : 1) : o->slices;
     if (want < 1 || want > egs::kMaxPhases)
         return fail(EGS_E_ARG, "slices must be 0 (auto) or 1..%d (got %d)", egs::kMaxPhases, o->slices);
     g.phases = 1;
@@ -561,7 +581,10 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
 
 // Transpose on the solver stream, then open the sparse round modes (ring[12] = 1).
 static hipError_t build_csc(egs_solver *s) {
-    hipError_t e = egs::launch_transpose(s->g, s->d_tile, s->d_fill, s->stream);
+    hipError_t e = egs::transpose_sorted_fits(s->g.n, s->g.m, s->tcounts_cap, s->m_cap)
+                       ? egs::launch_transpose_sorted(s->g, s->d_tcounts, s->d_ttot, s->d_tbstart, s->d_titems,
+                                                      s->stream)
+                       : egs::launch_transpose(s->g, s->d_tile, s->d_fill, s->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(s->d_ring + 12, s->h_probe + kPinOne, sizeof(int), hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) s->csc_built = true;

@@ -72,7 +72,7 @@ struct DevGame {
 
 constexpr int kMaxPhases = 16;             // successor slices of a phased dense round, at most
 constexpr int kDefaultPhases = 8;          // slices of 2 MB of values on the 2^22-vertex game
-constexpr int kPhaseMinN = 1 << 20;        // below this the value array fits the L2 anyway
+constexpr int kPhaseMinN = 1 << 20;        // up to this many vertices the values (4 MB) fit one XCD's L2
 constexpr int kSliceTile = 256;            // vertices per tile of the slice layout
 constexpr int kSliceWaves = kSliceTile / 64;
 constexpr int kMaxSliceDeg = 255;          // a vertex's edges into one slice fit a byte
@@ -102,6 +102,13 @@ struct LaunchShape {
 hipError_t launch_pack(const int *succ, const int *w, int2 *edges, int64_t m, hipStream_t st);
 hipError_t launch_unpack(const uint32_t *packed, int shift, int2 *edges, int64_t m, hipStream_t st);
 hipError_t launch_transpose(const DevGame &g, int *tile_sum, int *fill, hipStream_t st);
+// The transpose as a two-level stable counting sort (egs_transpose.hip): cpred in ascending
+// source order per column.  Buffers: counts[tiles * buckets], tot[4096], bstart[4097],
+// items[m].  transpose_sorted_fits says whether a game's sizes fit the buffers.
+int transpose_bshift(int n);
+bool transpose_sorted_fits(int n, int64_t m, int64_t counts_cap, int64_t items_cap);
+hipError_t launch_transpose_sorted(const DevGame &g, int *counts, int *tot, int *bstart, int2 *items, hipStream_t st);
+constexpr int kTrTileEdges = 1 << 16;        // edges per tile of the sorted transpose
 hipError_t launch_classify(const DevGame &g, int *counts, hipStream_t st);
 hipError_t launch_init(const DevGame &g, hipStream_t st);
 hipError_t launch_slices(const DevGame &g, hipStream_t st);

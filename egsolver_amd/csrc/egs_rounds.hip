@@ -578,9 +578,13 @@ constexpr int kSliceU = 8;
 __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *__restrict__ slen,
                                                            int *__restrict__ sbase, void *__restrict__ sedges,
                                                            unsigned long long *__restrict__ bigmask) {
+    // per thread, one column of counters indexed by slice (bank = thread: no conflicts):
+    // the vertex's edge counts, then the next position of its edges in each slice
+    __shared__ int s_cnt[kMaxPhases * kSliceTile];
     __shared__ int s_wt[kMaxPhases * kSliceWaves];
     const int P = g.phases;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int *col = s_cnt + threadIdx.x;
     const int64_t tiles = ((int64_t)g.n + kSliceTile - 1) / kSliceTile;
     for (int64_t T = blockIdx.x; T < tiles; T += gridDim.x) {
         const int64_t first = T * kSliceTile;
@@ -593,7 +597,7 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
         }
         const bool big = valid && e - s > g.big_deg;
         const bool mine = valid && !big;
-        int c[kMaxPhases] = {};
+        for (int q = 0; q < P; q++) col[q * kSliceTile] = 0;
         if (mine) {
             bool all_neg = true, any_neg = false;
             for (int k0 = s; k0 < e; k0 += kSliceU) {
@@ -603,9 +607,7 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
 #pragma unroll
                 for (int j = 0; j < kSliceU; j++) {
                     if (ew[j].x < 0) break;
-                    const int sl = ew[j].x >> g.slice_shift;
-#pragma unroll
-                    for (int q = 0; q < kMaxPhases; q++) c[q] += sl == q;
+                    col[(ew[j].x >> g.slice_shift) * kSliceTile]++;
                     all_neg &= ew[j].y < 0;
                     any_neg |= ew[j].y < 0;
                 }
@@ -616,14 +618,12 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
         const unsigned long long word = __ballot(big);
         if (lane == 0 && valid) bigmask[v >> 6] = word;
         uint8_t *len_out = slen + (size_t)T * P * kSliceTile;
-#pragma unroll
-        for (int q = 0; q < kMaxPhases; q++) {
-            if (q < P) {
-                len_out[q * kSliceTile + threadIdx.x] = (uint8_t)c[q];
-                int t = c[q];
-                for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
-                if (lane == 0) s_wt[q * kSliceWaves + wave] = t;
-            }
+        for (int q = 0; q < P; q++) {
+            const int c = col[q * kSliceTile];
+            len_out[q * kSliceTile + threadIdx.x] = (uint8_t)c;
+            int t = c;
+            for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
+            if (lane == 0) s_wt[q * kSliceWaves + wave] = t;
         }
         __syncthreads();
         // (slice, wave) sums scanned in that order: at most kMaxPhases * kSliceWaves = 64 items
@@ -637,9 +637,7 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
             }
         }
         __syncthreads();
-        int pos[kMaxPhases];
-#pragma unroll
-        for (int q = 0; q < kMaxPhases; q++) pos[q] = q < P ? s_wt[q * kSliceWaves + wave] + wave_excl_scan(c[q]) : 0;
+        for (int q = 0; q < P; q++) col[q * kSliceTile] = s_wt[q * kSliceWaves + wave] + wave_excl_scan(col[q * kSliceTile]);
         if (mine) {
             for (int k0 = s; k0 < e; k0 += kSliceU) {
                 int2 ew[kSliceU];
@@ -648,13 +646,9 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
 #pragma unroll
                 for (int j = 0; j < kSliceU; j++) {
                     if (ew[j].x < 0) break;
-                    const int sl = ew[j].x >> g.slice_shift;
-#pragma unroll
-                    for (int q = 0; q < kMaxPhases; q++) {
-                        if (sl != q) continue;
-                        if (g.sshift) ((uint32_t *)sedges)[pos[q]++] = ((uint32_t)ew[j].y << g.sshift) | (uint32_t)ew[j].x;
-                        else ((int2 *)sedges)[pos[q]++] = ew[j];
-                    }
+                    const int p = col[(ew[j].x >> g.slice_shift) * kSliceTile]++;
+                    if (g.sshift) ((uint32_t *)sedges)[p] = ((uint32_t)ew[j].y << g.sshift) | (uint32_t)ew[j].x;
+                    else ((int2 *)sedges)[p] = ew[j];
                 }
             }
         }

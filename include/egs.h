@@ -61,7 +61,7 @@ typedef struct egs_options {
                                src/dev_EG_alg.cu:376).  With a stop flag at most 4 rounds
                                are queued ahead of the host. */
     int32_t slices;         /* dense rounds in successor-range phases (their value gathers
-                               stay in L2): 0 = auto (8 phases when n >= 2^20 and the solver
+                               stay in L2): 0 = auto (8 phases when n > 2^20 and the solver
                                was created with n_cap >= 2^20), 1 = one sweep, 2..16 = at most
                                that many phases over slices of a power-of-two size.  Phased
                                games lift vertices above 255 edges with the wavefront kernels
