@@ -144,12 +144,12 @@ def single_game(E, g, args):
     s.load(g)
     shape = {"group": args.group, "grid": args.grid}
     if args.tune:
-        from egsolver_amd.autotune import tune_single
+        from egsolver_amd.autotune import tune_single, _TUNED_KEYS
         best, table = tune_single(g, solver=s)
-        shape.update({k: best[k] for k in ("group", "grid")})
-        log(f"tuned single-game shape {shape} over {len(table)} candidates")
+        shape = {k: best[k] for k in _TUNED_KEYS}
+        log(f"tuned single-game options {best} over {len(table)} candidates")
     # plain rounds (no proofs): the kernel measurement for the roofline
-    s.prepare(E.options(group=shape["group"], grid=shape["grid"], certify_every=-1))
+    s.prepare(E.options(certify_every=-1, **shape))
     _, a = s.iterate(args.warmup)
     _, b = s.iterate(args.rounds)
     d = {k: getattr(b, k) - getattr(a, k) for k in ("processed", "edges_relaxed", "lifted", "pred_visits", "phases")}
@@ -163,7 +163,7 @@ def single_game(E, g, args):
              "achieved_gbps": achieved}
     log(f"plain rounds: {per_launch_ms:.3f} ms/round, {plain['edges_per_s']:.3e} edges/s, {achieved:.0f} GB/s")
     # full solves to fixpoint (proofs on)
-    opt = E.options(group=shape["group"], grid=shape["grid"], certify_every=args.certify_every)
+    opt = E.options(certify_every=args.certify_every, **shape)
     for _ in range(args.warmup_solves):
         s.prepare(opt)
         s.iterate()
@@ -187,6 +187,7 @@ def single_game(E, g, args):
     t_c = time.perf_counter()
     cert = verify.certify(s, g, vals)
     cert["check_s"] = time.perf_counter() - t_c
+    csc_ms = transpose_ms(E, s)
     s.close()
     if not cert["ok"]:
         raise SystemExit(f"TOP-set certificate failed: {cert}")
@@ -199,7 +200,7 @@ def single_game(E, g, args):
             "value_checksum": int(np.bitwise_xor.reduce(vals.astype(np.uint32) * np.uint32(2654435761))),
             "fixpoint_violations": violations, "device_ms_prepare": last.ms_prepare,
             "device_ms_iterate": last.ms_iterate, "transpose_built": bool(last.csc_built),
-            "top_set_certificate": cert}
+            "transpose_forced_ms": csc_ms, "top_set_certificate": cert}
     return edges / t_all, t_all, info, plain, shape
 
 
@@ -210,7 +211,13 @@ def skewed_game(E, args):
     g = E.config_game(3, scale=args.scale)
     s = E.Solver(g.n, g.m)
     s.load(g)
-    s.prepare(E.options(certify_every=-1))
+    shape, tuned = {}, None
+    if args.tune:
+        from egsolver_amd.autotune import tune_single, _TUNED_KEYS
+        tuned, table = tune_single(g, solver=s)
+        shape = {k: tuned[k] for k in _TUNED_KEYS}
+        log(f"skewed game: tuned options {tuned} over {len(table)} candidates")
+    s.prepare(E.options(certify_every=-1, **shape))
     _, a = s.iterate(args.warmup)
     _, b = s.iterate(args.rounds // 4)
     d = {k: getattr(b, k) - getattr(a, k) for k in ("processed", "edges_relaxed", "lifted", "pred_visits", "phases")}
@@ -218,12 +225,12 @@ def skewed_game(E, args):
     per_round = ms_dev / max(d["phases"], 1)
     abytes = algorithmic_bytes(d) / max(d["phases"], 1)
     for _ in range(args.warmup_solves):
-        s.prepare(E.options())
+        s.prepare(E.options(**shape))
         s.iterate()
     times, st = [], None
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        s.prepare(E.options())
+        s.prepare(E.options(**shape))
         conv, st = s.iterate()
         times.append(time.perf_counter() - t0)
         if not conv:
@@ -231,6 +238,7 @@ def skewed_game(E, args):
     vals = s.values()
     from egsolver_amd import verify
     cert = verify.certify(s, g, vals)
+    csc_ms = transpose_ms(E, s)
     s.close()
     if not cert["ok"]:
         raise SystemExit(f"skewed game: TOP-set certificate failed: {cert}")
@@ -239,9 +247,10 @@ def skewed_game(E, args):
            "n": g.n, "m": g.m, "max_outdeg": int(np.diff(g.ptr).max()), "mg": g.mg,
            "solve_ms_mean": 1e3 * sum(times) / len(times), "solve_ms_min": 1e3 * min(times),
            "edges_relaxed_per_s": st.edges_relaxed * len(times) / sum(times), "rounds_per_solve": st.phases,
-           "proofs": st.certificates, "transpose_built": bool(st.csc_built), "device_ms_prepare": st.ms_prepare,
+           "proofs": st.certificates, "transpose_built": bool(st.csc_built), "transpose_forced_ms": csc_ms,
+           "device_ms_prepare": st.ms_prepare,
            "device_ms_iterate": st.ms_iterate, "top_vertices": cert["top_vertices"],
-           "top_set_certificate_ok": cert["ok"],
+           "top_set_certificate_ok": cert["ok"], "options": tuned,
            "plain_rounds": {"rounds": d["phases"], "device_ms_per_round": per_round,
                             "algorithmic_bytes_per_round": abytes,
                             "edges_per_s": d["edges_relaxed"] / (ms_dev * 1e-3),
@@ -250,6 +259,18 @@ def skewed_game(E, args):
     log(f"skewed game: m {g.m}, solve {out['solve_ms_mean']:.2f} ms, plain round {per_round:.3f} ms "
         f"({out['plain_rounds']['achieved_gbps']:.0f} GB/s)")
     return out
+
+
+def transpose_ms(E, s, reps=3):
+    """Device time of the transpose alone: prepares that build it up front (every round
+    sparse-capable, dense_pct > 100) less prepares that do not, medians of `reps`."""
+    def prep(opt):
+        ts = []
+        for _ in range(reps):
+            s.prepare(opt)
+            ts.append(s.stats().ms_prepare)
+        return float(np.median(ts))
+    return prep(E.options(dense_pct=101)) - prep(E.options())
 
 
 def gen_many(E, family, seed, count, scale_n=1.0, workers=16):
@@ -502,7 +523,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": plain["achieved_gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": plain["achieved_gbps"] / HBM_PEAK_GBPS, "traffic": traffic,
-            "kernel": "dense round: k_round + k_phase x 4 (successor-range phases) when n >= 2^20, else k_round",
+            "kernel": "dense round: k_round (mode decision) + k_dense_fused (8 successor-range phases in one launch)",
             "algorithmic_bytes_per_launch": plain["algorithmic_bytes_per_round"],
             "avg_launch_ms": plain["device_ms_per_round"],
             "copy_kernel_gbps": copy_gbps,

@@ -43,7 +43,8 @@ class EgsOptions(ctypes.Structure):
         ("max_phases", ctypes.c_int64),
         ("stop_flag", ctypes.c_void_p),
         ("slices", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("phase_wgs_per_cu", ctypes.c_int32),
+        ("graphs", ctypes.c_int32),
     ]
 
 
@@ -164,6 +165,9 @@ EGS_SIGNATURES = {
                                  _vp, _vp, _vp, _vp, ctypes.POINTER(EgsOptions), ctypes.POINTER(EgsStats)]),
     "egs_solver_count_violations": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "egs_solver_set_values": (ctypes.c_int, [_vp, _vp]),
+    "egs_solver_load_read_order": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp, _vp,
+                                                  ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _vp]),
+    "egs_solver_read_game": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "egs_solver_top_certificate": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, ctypes.POINTER(ctypes.c_int32)]),
     "egs_solve_noalloc": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _vp, _vp,
                                          _vp, _vp, ctypes.POINTER(EgsOptions), ctypes.POINTER(EgsStats), _vp]),

@@ -73,6 +73,7 @@ constexpr int kSparseWantRounds = 6;
 constexpr int kCertFirstCap = 64;     // longest-path passes per candidate at the first attempt
 constexpr int kCertCandidates = 8;    // gap thresholds tried per attempt (plus the last-round rise set)
 constexpr int kCertEvery = 8;         // default rounds before the first proof attempt
+constexpr int kGraphSizes = 6;        // captured batches of 1, 2, 4, ..., 32 rounds
 constexpr int kStopBatch = 4;         // rounds queued ahead of the host when a stop flag is polled
 
 double egs_now_ms() {
@@ -91,7 +92,7 @@ int pow2ceil(int x) {
 // workspace below is known at compile time.
 struct Layout {
     size_t ptr, edges, cptr, cpred, val, stamp, l0, l1, dbig, dhuge, sbig, shuge, lcnt, mode, ring, rounds, misc,
-        tile, fill, stats, statsum, sigma, h, hist, bfcnt, bits, hacc, hcnt, hbest, hflag, tcounts, ttot, tbstart,
+        tile, fill, stats, statsum, sigma, h, hist, bfcnt, bits, hacc, hcnt, hbest, hflag, tcounts, toffs, tscan,
         titems, slen, sbase, sedges, bigmask, total;
     int64_t tcounts_cap;
     int sub_cap;
@@ -147,9 +148,9 @@ constexpr Layout make_layout(int32_t n_cap, int64_t m_cap) {
     // and the edges as (source, destination) items grouped by bucket
     const int64_t nb_cap = std::min<int64_t>(4096, ((int64_t)n_cap + 2047) / 2048);
     L.tcounts_cap = ((m_cap + egs::kTrTileEdges - 1) / egs::kTrTileEdges) * nb_cap;
-    L.tcounts = take(4 * (size_t)L.tcounts_cap);
-    L.ttot = take(4 * 4096);
-    L.tbstart = take(4 * 4097);
+    L.tcounts = take(4 * ((size_t)L.tcounts_cap + 1));
+    L.toffs = take(4 * ((size_t)L.tcounts_cap + 1));
+    L.tscan = take(4 * ((size_t)L.tcounts_cap / 4096 + 2));
     L.titems = take(8 * m);
     // phased dense rounds (large games): per tile and slice the vertices' edge counts (a
     // byte each) and the wave bases, and the edges in (tile, slice, vertex) order
@@ -184,7 +185,7 @@ struct egs_solver {
     int *d_hacc = nullptr, *d_hcnt = nullptr;   // shared lifts of the vertices above huge_deg
     long long *d_hbest = nullptr;               // shared proof passes of the same vertices
     int *d_hflag = nullptr;
-    int *d_tcounts = nullptr, *d_ttot = nullptr, *d_tbstart = nullptr;   // sorted transpose
+    int *d_tcounts = nullptr, *d_toffs = nullptr, *d_tscan = nullptr;   // sorted transpose
     int2 *d_titems = nullptr;
     int64_t tcounts_cap = 0;
     uint8_t *d_slen = nullptr;                  // phased dense rounds (large n_cap)
@@ -204,6 +205,12 @@ struct egs_solver {
 
     egs::DevGame g{};
     egs::LaunchShape sh{};
+    // batches of 2^k rounds (k < kGraphSizes) captured once per prepare and replayed
+    hipGraphExec_t graph[kGraphSizes] = {};
+    egs::DevGame graph_g{};           // the game and shape the graphs were captured with
+    egs::LaunchShape graph_sh{};
+    bool use_graphs = false;
+    bool no_alloc = false;            // the no-allocation entry point's solver: never graphs
     bool loaded = false, prepared = false;
     int max_outdeg = 0;
     int64_t next_round = 1;
@@ -247,8 +254,8 @@ static void bind_layout(egs_solver *s, char *b, const Layout &L) {
     s->d_hbest = (long long *)(b + L.hbest);
     s->d_hflag = (int *)(b + L.hflag);
     s->d_tcounts = (int *)(b + L.tcounts);
-    s->d_ttot = (int *)(b + L.ttot);
-    s->d_tbstart = (int *)(b + L.tbstart);
+    s->d_toffs = (int *)(b + L.toffs);
+    s->d_tscan = (int *)(b + L.tscan);
     s->d_titems = (int2 *)(b + L.titems);
     s->tcounts_cap = L.tcounts_cap;
     if (L.phased) {
@@ -289,6 +296,42 @@ static int64_t list_total(const egs_solver *s, const int *c) {
     int64_t t = 0;
     for (int k = 0; k < egs::kSub; k++) t += std::min(c[k], s->sub_cap);
     return t + c[egs::kSub];
+}
+
+static void drop_graphs(egs_solver *s) {
+    for (auto &x : s->graph)
+        if (x) {
+            (void)hipGraphExecDestroy(x);
+            x = nullptr;
+        }
+}
+
+// One batch of `rounds` rounds (k_advance last) on the solver stream.
+static hipError_t launch_batch(egs_solver *s, int rounds) {
+    for (int j = 0; j < rounds; j++) {
+        hipError_t e = egs::launch_round(s->g, s->sh, j, s->stream);
+        if (e != hipSuccess) return e;
+    }
+    return egs::launch_advance(s->g, rounds, s->stream);
+}
+
+// The same batch as a graph of 2^k rounds, captured on first use after a prepare.
+static hipError_t replay_batch(egs_solver *s, int k) {
+    if (!s->graph[k]) {
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        const hipError_t le = launch_batch(s, 1 << k);
+        e = hipStreamEndCapture(s->stream, &gr);
+        if (le != hipSuccess) e = le;
+        if (e == hipSuccess) e = hipGraphInstantiate(&s->graph[k], gr, nullptr, nullptr, 0);
+        if (gr) (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) {
+            s->graph[k] = nullptr;
+            return e;
+        }
+    }
+    return hipGraphLaunch(s->graph[k], s->stream);
 }
 
 extern "C" {
@@ -353,6 +396,7 @@ int egs_solver_destroy(egs_solver *s) {
     if (!s) return EGS_OK;
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    drop_graphs(s);
     if (s->block) (void)hipFree(s->block);
     if (s->h_probe) (void)hipHostFree(s->h_probe);
     for (auto &e : s->ev_probe) if (e) (void)hipEventDestroy(e);
@@ -477,6 +521,93 @@ int egs_solver_load_packed(egs_solver *s, int32_t n, int32_t n0, int64_t m, int3
     return EGS_OK;
 }
 
+int egs_solver_load_read_order(egs_solver *s, int mem, int32_t n, int64_t m, const uint8_t *owner,
+                               const int32_t *rptr, const int32_t *rsucc, const int32_t *rw, int32_t *n0_out,
+                               int32_t *mg_out, int32_t *ext_of_int) {
+    if (!s) return fail(EGS_E_ARG, "null solver");
+    if (n < 1 || m < 1 || !owner || !rptr || !rsucc || !rw || (mem != EGS_MEM_HOST && mem != EGS_MEM_DEVICE))
+        return fail(EGS_E_ARG, "egs_solver_load_read_order: bad game (n=%d m=%lld)", n, (long long)m);
+    if (n > s->n_cap || m > s->m_cap)
+        return fail(EGS_E_CAPACITY, "game (n=%d, m=%lld) exceeds solver capacity (%d, %lld)", n, (long long)m,
+                    s->n_cap, (long long)s->m_cap);
+    HIP_TRY(hipSetDevice(s->device));
+    egs::ReadOrderBuffers b{};
+    if (mem == EGS_MEM_HOST) {
+        // one upload into work space that prepare and the rounds rewrite later
+        uint8_t *d_owner = (uint8_t *)s->d_stamp;
+        int *d_rsucc = (int *)s->d_titems, *d_rw = (int *)s->d_titems + m;
+        HIP_TRY(hipMemcpyAsync(d_owner, owner, (size_t)n, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->d_cptr, rptr, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(d_rsucc, rsucc, sizeof(int) * (size_t)m, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(d_rw, rw, sizeof(int) * (size_t)m, hipMemcpyHostToDevice, s->stream));
+        b.owner = d_owner;
+        b.rptr = s->d_cptr;
+        b.rsucc = d_rsucc;
+        b.rw = d_rw;
+    } else {
+        b.owner = owner;
+        b.rptr = rptr;
+        b.rsucc = rsucc;
+        b.rw = rw;
+    }
+    HIP_TRY(egs::launch_read_order_check(b.rptr, b.rsucc, n, m, s->d_misc + 8, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_probe + kPinDeg, s->d_misc + 8, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const int bad = s->h_probe[kPinDeg];
+    if (bad & 1) return fail(EGS_E_ARG, "read-order offsets must run from 0 to m");
+    if (bad & 2) return fail(EGS_E_ARG, "a vertex has no out-edge");
+    if (bad & 4) return fail(EGS_E_ARG, "a successor is out of range");
+    b.flag0 = s->d_sigma;
+    b.pre0 = s->d_fill;
+    b.rev = s->d_dbig;
+    b.ext_of_int = s->d_sbig;
+    b.deg = s->d_dhuge;
+    b.scan_tiles = s->d_tile;
+    unsigned long long *d_mg = (unsigned long long *)s->d_misc;
+    HIP_TRY(egs::launch_internalise(b, n, s->d_ptr, s->d_edges, d_mg, s->stream));
+    unsigned long long *h_mg = (unsigned long long *)(s->h_probe + kPinViol);
+    HIP_TRY(hipMemcpyAsync(h_mg, d_mg, sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_probe + kPinDeg + 1, s->d_fill + n, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    if (ext_of_int)
+        HIP_TRY(hipMemcpyAsync(ext_of_int, s->d_sbig, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(egs::launch_degree_stats(s->d_ptr, n, s->d_misc + 2, s->stream));
+    egs::DevGame tmp{};
+    tmp.edges = s->d_edges;
+    tmp.m = m;
+    HIP_TRY(egs::launch_abs_wmax(tmp, s->d_misc + 3, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->h_probe + kPinDeg + 2, s->d_misc + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (*h_mg > 0x7fffffffULL) return fail(EGS_E_ARG, "Error too high weights --> overflow (MG past INT32_MAX)");
+    const int32_t n0 = s->h_probe[kPinDeg + 1], mg = (int32_t)*h_mg;
+    s->max_outdeg = s->h_probe[kPinDeg + 2];
+    s->wmax = s->h_probe[kPinDeg + 3];
+    bind_game(s, n, n0, m, mg);
+    if (n0_out) *n0_out = n0;
+    if (mg_out) *mg_out = mg;
+    return EGS_OK;
+}
+
+int egs_solver_read_game(egs_solver *s, int32_t *ptr, int32_t *succ, int32_t *w) {
+    if (!s || !s->loaded || !ptr || !succ || !w) return fail(EGS_E_STATE, "egs_solver_read_game: nothing loaded");
+    HIP_TRY(hipSetDevice(s->device));
+    const int64_t m = s->g.m;
+    int2 *tmp = (int2 *)malloc(sizeof(int2) * (size_t)m);
+    if (!tmp) return fail(EGS_E_NOMEM, "host buffer of %lld edges", (long long)m);
+    hipError_t e1 = hipMemcpyAsync(ptr, s->d_ptr, sizeof(int) * ((size_t)s->g.n + 1), hipMemcpyDeviceToHost, s->stream);
+    hipError_t e2 = hipMemcpyAsync(tmp, s->d_edges, sizeof(int2) * (size_t)m, hipMemcpyDeviceToHost, s->stream);
+    hipError_t e3 = hipStreamSynchronize(s->stream);
+    if (e1 == hipSuccess && e2 == hipSuccess && e3 == hipSuccess)
+        for (int64_t k = 0; k < m; k++) {
+            succ[k] = tmp[k].x;
+            w[k] = tmp[k].y;
+        }
+    free(tmp);
+    HIP_TRY(e1);
+    HIP_TRY(e2);
+    HIP_TRY(e3);
+    return EGS_OK;
+}
+
 // Point the device-game view at the solver's arrays for a freshly loaded game.
 static void bind_game(egs_solver *s, int32_t n, int32_t n0, int64_t m, int32_t mg) {
     egs::DevGame &g = s->g;
@@ -532,6 +663,8 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     // with a stop flag at most kStopBatch rounds are queued ahead of the host, so a stop
     // takes effect within a few rounds
     if (o->stop_flag) s->batch_rounds = std::min(s->batch_rounds, kStopBatch);
+    if (o->graphs < -1 || o->graphs > 1) return fail(EGS_E_ARG, "graphs must be -1, 0 or 1 (got %d)", o->graphs);
+    s->use_graphs = o->graphs > 0 && !o->stop_flag && !s->no_alloc;
     s->phase_cap = egs_max_loop_c(g.m, g.n, g.mg, o->max_phases);
     s->stop_flag = o->stop_flag;
     s->cert_every = o->certify_every == 0 ? kCertEvery : (o->certify_every < 0 ? 0 : o->certify_every);
@@ -569,7 +702,8 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
     // The phased kernels run about as many workgroups as the device holds at once (eight of
     // 256 threads per CU), each over a fixed set of tiles whose partial results sit in LDS.
     const int64_t tiles = ((int64_t)g.n + egs::kSliceTile - 1) / egs::kSliceTile;
-    g.phase_grid = (int)std::min<int64_t>(tiles, 8 * (int64_t)std::max(1, s->cus));
+    const int wpc = o->phase_wgs_per_cu > 0 ? std::min(o->phase_wgs_per_cu, 32) : 8;
+    g.phase_grid = (int)std::min<int64_t>(tiles, wpc * (int64_t)std::max(1, s->cus));
     g.phase_tiles = (int)((tiles + g.phase_grid - 1) / g.phase_grid);
     if (g.phases > 1 && g.phase_tiles * egs::kSliceTile * 12 > 96 * 1024) {
         // keep the closure kernel's partial state (12 bytes a vertex) within 96 KB of LDS
@@ -582,7 +716,7 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
 // Transpose on the solver stream, then open the sparse round modes (ring[12] = 1).
 static hipError_t build_csc(egs_solver *s) {
     hipError_t e = egs::transpose_sorted_fits(s->g.n, s->g.m, s->tcounts_cap, s->m_cap)
-                       ? egs::launch_transpose_sorted(s->g, s->d_tcounts, s->d_ttot, s->d_tbstart, s->d_titems,
+                       ? egs::launch_transpose_sorted(s->g, s->d_tcounts, s->d_toffs, s->d_tscan, s->d_titems,
                                                       s->stream)
                        : egs::launch_transpose(s->g, s->d_tile, s->d_fill, s->stream);
     if (e == hipSuccess)
@@ -596,6 +730,12 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     HIP_TRY(hipSetDevice(s->device));
     int rc = pick_shape(s, opt);
     if (rc) return rc;
+    // graphs hold the launch arguments they were captured with: kept while those stay the same
+    if (memcmp(&s->graph_g, &s->g, sizeof(s->g)) != 0 || memcmp(&s->graph_sh, &s->sh, sizeof(s->sh)) != 0) {
+        drop_graphs(s);
+        memcpy(&s->graph_g, &s->g, sizeof(s->g));
+        memcpy(&s->graph_sh, &s->sh, sizeof(s->sh));
+    }
     double t0 = 0;
     HIP_TRY(mark_start(s, &t0));
     HIP_TRY(hipMemsetAsync(s->d_stats, 0, sizeof(unsigned long long) * 4 * egs::kStatSlots, s->stream));
@@ -616,6 +756,7 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     HIP_TRY(egs::launch_slices(s->g, s->stream));
     HIP_TRY(egs::launch_init(s->g, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_ring + 11, 0, sizeof(int), s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_ring + 14, s->h_probe + kPinOne, sizeof(int), hipMemcpyHostToDevice, s->stream));
     float ms = 0.f;
     HIP_TRY(mark_stop(s, t0, &ms));
     HIP_TRY(hipMemcpyAsync(s->h_probe + kPinInit, s->d_lcnt + 1 * egs::kListCounters,
@@ -703,14 +844,23 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
         }
         if (accel) nb = (int)std::max<int64_t>(1, std::min<int64_t>(nb, s->next_cert - (s->next_round - 1)));
         int k = 0;
-        for (; k < nb; k++) {
-            // the stop flag is polled before every round, as the reference polls
-            // timeout_expired once per round (src/dev_EG_alg.cu:376)
-            if (s->stop_flag && *s->stop_flag) break;
-            HIP_TRY(egs::launch_round(s->g, s->sh, (int)s->next_round, s->stream));
-            s->next_round++;
+        if (s->use_graphs) {
+            for (int b = kGraphSizes - 1; b >= 0; b--)
+                while (nb - k >= (1 << b)) {
+                    HIP_TRY(replay_batch(s, b));
+                    k += 1 << b;
+                }
+        } else {
+            for (; k < nb; k++) {
+                // the stop flag is polled before every round, as the reference polls
+                // timeout_expired once per round (src/dev_EG_alg.cu:376)
+                if (s->stop_flag && *s->stop_flag) break;
+                HIP_TRY(egs::launch_round(s->g, s->sh, k, s->stream));
+            }
+            if (k > 0) HIP_TRY(egs::launch_advance(s->g, k, s->stream));
         }
         if (k == 0) break;
+        s->next_round += k;
         launched += k;
         const int slot = (pending + 1) & 1;
         HIP_TRY(hipMemcpyAsync(s->h_probe + kPinProbe + slot * 128,
@@ -868,6 +1018,7 @@ static int noalloc_setup(int dev) {
         return fail(EGS_E_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
     egs_solver &s = sl.s;
     s.device = dev;
+    s.no_alloc = true;
     if (hipDeviceGetAttribute(&s.cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) s.cus = 256;
     s.n_cap = kNoAllocN;
     s.m_cap = kNoAllocM;

@@ -114,6 +114,7 @@ __device__ __forceinline__ int game_sweep_part(const GameDesc &b, const GameVals
             s = b.ptr[v];
             e = b.ptr[v + 1];
             old = gv<SPLIT>(V, v);
+            if (old == EGS_TOP) e = s;          // TOP is final: nothing to gather
         }
         const bool owner1 = v >= b.n0;
         int key = EGS_TOP;

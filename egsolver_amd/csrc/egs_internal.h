@@ -36,7 +36,8 @@ struct DevGame {
     int *ring;            // [3..5] dbig counters, [6..8] dhuge counters, [9] nsbig, [10] nshuge,
                           // [11] a round that must sweep all vertices (after a TOP proof),
                           // [12] 1 once the transpose exists (sparse rounds allowed),
-                          // [13] rounds that would have gone sparse without it
+                          // [13] rounds that would have gone sparse without it,
+                          // [14] the absolute number of the next batch's first round
     const int *sbig;      // static: vertices with big_deg < deg <= huge_deg
     const int *shuge;     // static: vertices with deg > huge_deg
     int big_deg, huge_deg;
@@ -62,7 +63,8 @@ struct DevGame {
     int sshift;
     const unsigned long long *bigmask;   // bit v: vertex v is above big_deg (lifted by the wave/block kernels)
     int phase_grid;       // workgroups of the phased kernels (about what the device holds at once)
-    int phase_tiles;      // tiles of the slice layout per workgroup, at most
+    int phase_tiles;      // tiles of the slice layout per workgroup, at most 32 (96 KB of LDS
+                          // at 12 bytes per vertex; the dense kernel keeps a 32-bit mask of them)
     int *hacc, *hcnt;     // n each: partial keys and finished chunks of the vertices above
                           // huge_deg (TOP and 0 between rounds)
     long long *hbest;     // n: partial longest-path candidates of the vertices above huge_deg
@@ -103,17 +105,33 @@ hipError_t launch_pack(const int *succ, const int *w, int2 *edges, int64_t m, hi
 hipError_t launch_unpack(const uint32_t *packed, int shift, int2 *edges, int64_t m, hipStream_t st);
 hipError_t launch_transpose(const DevGame &g, int *tile_sum, int *fill, hipStream_t st);
 // The transpose as a two-level stable counting sort (egs_transpose.hip): cpred in ascending
-// source order per column.  Buffers: counts[tiles * buckets], tot[4096], bstart[4097],
-// items[m].  transpose_sorted_fits says whether a game's sizes fit the buffers.
+// source order per column.  Buffers: counts and offs of transpose_counts_len + 1 ints each,
+// scan_tiles of transpose_counts_len / 4096 + 2 ints, items[m].  transpose_sorted_fits
+// says whether a game's sizes fit the buffers.
 int transpose_bshift(int n);
+int64_t transpose_counts_len(int n, int64_t m);
 bool transpose_sorted_fits(int n, int64_t m, int64_t counts_cap, int64_t items_cap);
-hipError_t launch_transpose_sorted(const DevGame &g, int *counts, int *tot, int *bstart, int2 *items, hipStream_t st);
-constexpr int kTrTileEdges = 1 << 16;        // edges per tile of the sorted transpose
+hipError_t launch_transpose_sorted(const DevGame &g, int *counts, int *offs, int *scan_tiles, int2 *items,
+                                   hipStream_t st);
+hipError_t launch_exclusive_scan(const int *in, int n, int *tile_sum, int *out, hipStream_t st);
+constexpr int kTrTileEdges = 1 << 14;        // edges per tile of the sorted transpose
+
+// A game in read order on the device and the work arrays of its renaming (egs_ingest.hip).
+struct ReadOrderBuffers {
+    const uint8_t *owner;                      // n: 0 = owner 0 (token MAX), else owner 1
+    const int *rptr, *rsucc, *rw;              // read-order CSR, successors as read indices
+    int *flag0, *pre0, *rev, *ext_of_int, *deg, *scan_tiles;   // n (+1) each; scan_tiles n/4096 + 1
+};
+hipError_t launch_read_order_check(const int *rptr, const int *rsucc, int n, int64_t m, int *bad, hipStream_t st);
+hipError_t launch_internalise(const ReadOrderBuffers &b, int n, int *ptr, int2 *edges, unsigned long long *mg,
+                              hipStream_t st);
 hipError_t launch_classify(const DevGame &g, int *counts, hipStream_t st);
 hipError_t launch_init(const DevGame &g, hipStream_t st);
 hipError_t launch_slices(const DevGame &g, hipStream_t st);
 int pack_shift(int n, int wmax);
-hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int r, hipStream_t st);
+// Round j of a batch (absolute round ring[14] + j); launch_advance ends the batch.
+hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int j, hipStream_t st);
+hipError_t launch_advance(const DevGame &g, int rounds, hipStream_t st);
 hipError_t launch_sum_stats(const DevGame &g, int nslots, unsigned long long *out, hipStream_t st);
 hipError_t launch_degree_stats(const int *ptr, int n, int *out, hipStream_t st);
 hipError_t launch_violations(const DevGame &g, unsigned long long *out, hipStream_t st);

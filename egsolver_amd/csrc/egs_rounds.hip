@@ -336,6 +336,7 @@ __device__ __forceinline__ void body_dense(const DevGame &g, const Stage &st, in
             e = g.ptr[v + 1];
             if (e - s > g.big_deg) { valid = false; s = e = 0; }   // static wave/block lists
             else old = g.val[v];
+            if (old == EGS_TOP) e = s;          // TOP is final: nothing to gather
         }
         const int t = lift_value<G>(g, s, e, gl, v >= g.n0);
         const bool lifted = valid && old < t;
@@ -435,7 +436,8 @@ __device__ __forceinline__ void body_lifted(const DevGame &g, const Stage &st, c
 }
 
 template <int G>
-__global__ __launch_bounds__(256) void k_round(DevGame g, int r) {
+__global__ __launch_bounds__(256) void k_round(DevGame g, int j) {
+    const int r = g.ring[14] + j;
     __shared__ int pre[kSub + 2];
     __shared__ int sbuf[kLdsCap];
     __shared__ int scnt, smode;
@@ -499,7 +501,8 @@ __device__ __forceinline__ int wave_excl_scan(int x) {
 constexpr int kPhaseU = 2;   // edges in flight per lane
 
 template <bool PACK>
-__global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int r) {
+__global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int j) {
+    const int r = g.ring[14] + j;
     extern __shared__ int keys[];                 // [tile of this workgroup][kSliceTile]
     __shared__ int sbuf[kLdsCap];
     __shared__ int scnt;
@@ -514,6 +517,12 @@ __global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int r) {
     const int64_t W = gridDim.x;
     const int K = blockIdx.x < tiles ? (int)((tiles - blockIdx.x + W - 1) / W) : 0;
     const int wave = threadIdx.x >> 6;
+    // vertices already at TOP (final) skip their gathers; at most 32 tiles per workgroup
+    uint32_t settled = 0;
+    for (int k = 0; k < K; k++) {
+        const int64_t v = (blockIdx.x + k * W) * kSliceTile + threadIdx.x;
+        if (v < g.n && g.val[v] == EGS_TOP) settled |= 1u << k;
+    }
     uint64_t edges = 0;
     for (int s = 0; s < P; s++) {
         for (int k = 0; k < K; k++) {
@@ -521,7 +530,7 @@ __global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int r) {
             const size_t seg = (size_t)T * P + s;
             const int len = g.slen[seg * kSliceTile + threadIdx.x];
             const int b = g.sbase[seg * kSliceWaves + wave] + wave_excl_scan(len);
-            const int e = b + len;
+            const int e = ((settled >> k) & 1u) ? b : b + len;   // the prefix counts every lane's edges
             const bool owner1 = T * kSliceTile + threadIdx.x >= g.n0;
             int key = EGS_TOP;
             // kPhaseU edges per lane at a time: their loads, then their gathers, are all in
@@ -536,7 +545,7 @@ __global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int r) {
                 for (int j = 0; j < kPhaseU; j++)
                     if (k0 + j < e) key = min(key, egs_key(egs_ominus(x[j], w[j], g.mg), owner1));
             }
-            edges += (uint32_t)len;
+            edges += (uint32_t)(e - b);
             int &slot = keys[k * kSliceTile + threadIdx.x];
             slot = s == 0 ? key : min(slot, key);
         }
@@ -545,7 +554,7 @@ __global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int r) {
         const int64_t T = blockIdx.x + k * W;
         const int v = (int)(T * kSliceTile + threadIdx.x);
         bool lifted = false;
-        if (v < g.n && !((g.bigmask[v >> 6] >> (v & 63)) & 1ull)) {
+        if (v < g.n && !((settled >> k) & 1u) && !((g.bigmask[v >> 6] >> (v & 63)) & 1ull)) {
             const int t = egs_unkey(keys[k * kSliceTile + threadIdx.x], v >= g.n0);
             const int old = g.val[v];
             acc.processed++;
@@ -658,7 +667,8 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
 
 // One wavefront per vertex: the static big list in dense rounds, the deferred list in
 // sparse rounds.
-__global__ __launch_bounds__(256) void k_lift_wave(DevGame g, int r) {
+__global__ __launch_bounds__(256) void k_lift_wave(DevGame g, int j) {
+    const int r = g.ring[14] + j;
     __shared__ int sbuf[kLdsCap];
     __shared__ int scnt;
     const int mode = g.mode[r % 3];
@@ -679,6 +689,7 @@ __global__ __launch_bounds__(256) void k_lift_wave(DevGame g, int r) {
         const int v = items[i];
         const int s = g.ptr[v], e = g.ptr[v + 1];
         const int old = g.val[v];
+        if (old == EGS_TOP) continue;           // final; the whole wave skips it
         const int t = lift_value<64>(g, s, e, lane, v >= g.n0);
         const bool lifted = old < t;
         if (lane == 0) {
@@ -701,7 +712,8 @@ __global__ __launch_bounds__(256) void k_lift_wave(DevGame g, int r) {
 // walking a hub of 70K edges alone made this kernel most of a skewed game's round.
 constexpr int kHugeChunk = 2048;
 
-__global__ __launch_bounds__(256) void k_lift_block(DevGame g, int r) {
+__global__ __launch_bounds__(256) void k_lift_block(DevGame g, int j) {
+    const int r = g.ring[14] + j;
     __shared__ int sbuf[kLdsCap];
     __shared__ int scnt;
     __shared__ int wmin[4];
@@ -728,6 +740,9 @@ __global__ __launch_bounds__(256) void k_lift_block(DevGame g, int r) {
         const int chunks = (e - s + kHugeChunk - 1) / kHugeChunk;
         const int contrib = min(per, chunks);
         if (part >= contrib) continue;                   // the whole workgroup: u is shared
+        // a TOP value is final; every contributor sees it (only v's own last contributor
+        // changes v's value in a round, after all others have counted in)
+        if (g.val[v] == EGS_TOP) continue;
         const bool owner1 = v >= g.n0;
         int key = EGS_TOP;
         for (int c = part; c < chunks; c += contrib) {
@@ -787,25 +802,38 @@ hipError_t launch_init(const DevGame &g, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int r, hipStream_t st) {
+// The round kernels of a batch take their offset j in the batch; the absolute round number
+// is ring[14] + j, and k_advance moves ring[14] past the batch after its last kernel.  A
+// batch's launches are then the same every time, so a batch can be captured as a graph once
+// and replayed.
+__global__ void k_advance(DevGame g, int rounds) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) g.ring[14] += rounds;
+}
+
+hipError_t launch_advance(const DevGame &g, int rounds, hipStream_t st) {
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, g, rounds);
+    return hipGetLastError();
+}
+
+hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int j, hipStream_t st) {
     switch (sh.group) {
-    case 1: hipLaunchKernelGGL(k_round<1>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
-    case 2: hipLaunchKernelGGL(k_round<2>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
-    case 4: hipLaunchKernelGGL(k_round<4>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
-    case 8: hipLaunchKernelGGL(k_round<8>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
-    case 16: hipLaunchKernelGGL(k_round<16>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
-    case 32: hipLaunchKernelGGL(k_round<32>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
-    default: hipLaunchKernelGGL(k_round<64>, dim3(sh.grid), dim3(256), 0, st, g, r); break;
+    case 1: hipLaunchKernelGGL(k_round<1>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
+    case 2: hipLaunchKernelGGL(k_round<2>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
+    case 4: hipLaunchKernelGGL(k_round<4>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
+    case 8: hipLaunchKernelGGL(k_round<8>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
+    case 16: hipLaunchKernelGGL(k_round<16>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
+    case 32: hipLaunchKernelGGL(k_round<32>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
+    default: hipLaunchKernelGGL(k_round<64>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
     }
     if (g.phases > 1) {
         const size_t lds = (size_t)g.phase_tiles * kSliceTile * sizeof(int);
         if (g.sshift)
-            hipLaunchKernelGGL(k_dense_fused<true>, dim3(g.phase_grid), dim3(256), lds, st, g, r);
+            hipLaunchKernelGGL(k_dense_fused<true>, dim3(g.phase_grid), dim3(256), lds, st, g, j);
         else
-            hipLaunchKernelGGL(k_dense_fused<false>, dim3(g.phase_grid), dim3(256), lds, st, g, r);
+            hipLaunchKernelGGL(k_dense_fused<false>, dim3(g.phase_grid), dim3(256), lds, st, g, j);
     }
-    if (sh.use_wave) hipLaunchKernelGGL(k_lift_wave, dim3(sh.wave_grid), dim3(256), 0, st, g, r);
-    if (sh.use_block) hipLaunchKernelGGL(k_lift_block, dim3(sh.block_grid), dim3(256), 0, st, g, r);
+    if (sh.use_wave) hipLaunchKernelGGL(k_lift_wave, dim3(sh.wave_grid), dim3(256), 0, st, g, j);
+    if (sh.use_block) hipLaunchKernelGGL(k_lift_block, dim3(sh.block_grid), dim3(256), 0, st, g, j);
     return hipGetLastError();
 }
 

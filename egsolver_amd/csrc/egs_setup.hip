@@ -267,6 +267,17 @@ hipError_t launch_pack(const int *succ, const int *w, int2 *edges, int64_t m, hi
     return hipGetLastError();
 }
 
+// Exclusive scan of in[0..n) into out[0..n] (out[n] = total); in == out is allowed (each
+// tile is read whole into LDS before it is written).  tile_sum holds ceil(n / SCAN_TILE).
+hipError_t launch_exclusive_scan(const int *in, int n, int *tile_sum, int *out, hipStream_t st) {
+    const int tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles == 0) return hipMemsetAsync(out, 0, sizeof(int), st);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(SCAN_THREADS), 0, st, in, n, tile_sum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_THREADS), 0, st, tile_sum, tiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(SCAN_THREADS), 0, st, in, n, tile_sum, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_transpose(const DevGame &g, int *tile_sum, int *fill, hipStream_t st) {
     const int n = g.n;
     const int64_t m = g.m;

@@ -16,10 +16,11 @@ from .games import Game
 
 
 def options(group=0, big_deg=0, huge_deg=0, grid=0, batch_rounds=0, max_phases=-1, stop_flag=None, dense_pct=0,
-            certify_every=0, cert_max_passes=0, slices=0):
+            certify_every=0, cert_max_passes=0, slices=0, phase_wgs_per_cu=0, graphs=0):
     o = N.EgsOptions(group=group, big_deg=big_deg, huge_deg=huge_deg, grid=grid,
                      batch_rounds=batch_rounds, dense_pct=dense_pct, certify_every=certify_every,
-                     cert_max_passes=cert_max_passes, max_phases=max_phases, stop_flag=None, slices=slices)
+                     cert_max_passes=cert_max_passes, max_phases=max_phases, stop_flag=None, slices=slices,
+                     phase_wgs_per_cu=phase_wgs_per_cu, graphs=graphs)
     if stop_flag is not None:
         o.stop_flag = ctypes.cast(ctypes.pointer(stop_flag), ctypes.c_void_p)
     return o
@@ -59,6 +60,30 @@ class Solver:
         N.check(N.egs().egs_solver_load(self.h, 0, g.n, g.n0, g.m, g.mg, ptr.ctypes.data, succ.ctypes.data,
                                         w.ctypes.data), "egs_solver_load")
         self.n, self.m = g.n, g.m
+
+    def load_read_order(self, owner, rptr, rsucc, rw):
+        """A game in read order (external id = read index, e.g. games.generate_read_order);
+        the owner partition, internal CSR and MG are computed on the device.  Returns
+        (n0, mg, ext_of_int)."""
+        owner = np.ascontiguousarray(owner, np.uint8)
+        rptr, rsucc, rw = (np.ascontiguousarray(a, np.int32) for a in (rptr, rsucc, rw))
+        n, m = len(owner), int(rptr[-1])
+        n0, mg = ctypes.c_int32(0), ctypes.c_int32(0)
+        ext = np.empty(n, np.int32)
+        N.check(N.egs().egs_solver_load_read_order(self.h, 0, n, m, owner.ctypes.data, rptr.ctypes.data,
+                                                   rsucc.ctypes.data, rw.ctypes.data, ctypes.byref(n0),
+                                                   ctypes.byref(mg), ext.ctypes.data), "egs_solver_load_read_order")
+        self.n, self.m = n, m
+        return int(n0.value), int(mg.value), ext
+
+    def game(self):
+        """The loaded game's internal CSR: (ptr, succ, w)."""
+        ptr = np.empty(self.n + 1, np.int32)
+        succ = np.empty(self.m, np.int32)
+        w = np.empty(self.m, np.int32)
+        N.check(N.egs().egs_solver_read_game(self.h, ptr.ctypes.data, succ.ctypes.data, w.ctypes.data),
+                "egs_solver_read_game")
+        return ptr, succ, w
 
     def load_device(self, n, n0, m, mg, d_ptr, d_succ, d_w):
         """Arrays already resident in device memory (integer device addresses)."""

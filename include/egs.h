@@ -66,7 +66,13 @@ typedef struct egs_options {
                                that many phases over slices of a power-of-two size.  Phased
                                games lift vertices above 255 edges with the wavefront kernels
                                (big_deg is capped at 255). */
-    int32_t reserved;
+    int32_t phase_wgs_per_cu; /* workgroups per compute unit of the phased kernels (their
+                               grid is about what the device holds at once); 0 = auto (8) */
+    int32_t graphs;         /* batches of rounds replayed as HIP graphs captured once per
+                               prepare (batches of 1,2,4,...,32 rounds): 1 = on, -1 = off
+                               (every kernel launched), 0 = auto.  Off while a stop flag
+                               is set (it is polled before each round) and in
+                               egs_solve_noalloc (instantiating a graph allocates). */
 } egs_options;
 
 typedef struct egs_stats {
@@ -143,6 +149,22 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt);
 int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st);
 
 int egs_solver_stats(egs_solver *s, egs_stats *st);
+/* No reference counterpart.  Load a game in read order, as a reader or generator produces
+ * it (external id = read index): owner[n] (0 = token MAX = owner 0, else owner 1), rptr[n+1],
+ * rsucc[m] (read indices), rw[m]; host or device arrays per `mem`.  The owner partition
+ * (owner-0 vertices first, each block in read order: src/egsolver.c:146-214 with the
+ * default comparator under glibc's stable qsort), the internal CSR (src/egsolver.c:253-281)
+ * and MG (src/egsolver.c:283-304) are computed on the device after one upload.  The input is
+ * checked on the device (offsets from 0 to m, an edge per vertex, successors in range;
+ * EGS_E_ARG otherwise, also when MG passes INT32_MAX).  *n0_out and *mg_out get the
+ * internal owner-0 count and MG; ext_of_int[n] (may be NULL) the read index of each
+ * internal vertex. */
+int egs_solver_load_read_order(egs_solver *s, int mem, int32_t n, int64_t m, const uint8_t *owner,
+                               const int32_t *rptr, const int32_t *rsucc, const int32_t *rw, int32_t *n0_out,
+                               int32_t *mg_out, int32_t *ext_of_int);
+/* No reference counterpart.  The loaded game's internal CSR back on the host: ptr[n+1],
+ * succ[m], w[m]. */
+int egs_solver_read_game(egs_solver *s, int32_t *ptr, int32_t *succ, int32_t *w);
 /* No reference counterpart.  Load a game already resident on the device in the 32-bit
  * packed edge layout of the batched and serving paths (edge e = (w << shift) | succ,
  * 1 <= shift <= 30), with its max |w| and max out-degree known; the CSR is trusted to

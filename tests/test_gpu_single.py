@@ -401,3 +401,51 @@ def test_stop_flag_from_another_thread_then_resume(tmp_path):
         _say(f"resumed: {st2.phases} rounds in total, {time.perf_counter() - t2:.2f} s more")
         assert conv and st2.phases > st.phases
         assert np.array_equal(s.values(), lfp)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_graph_replayed_rounds_equal_launched_rounds(oracle, seed):
+    """Batches of rounds replayed as captured HIP graphs give the values of the same rounds
+    launched kernel by kernel, across repeated solves of one prepared solver (graphs reused)
+    and re-prepares with other shapes (graphs re-captured): lane, wavefront and workgroup
+    kernels, sparse rounds and proofs.  Round counts and lift counts are not compared: a
+    round reads values other workgroups raised in the same round, so they vary from run to
+    run (by a few lifts) with either launch form."""
+    import egsolver_amd as E
+    rng = np.random.default_rng(50 + seed)
+    n = int(rng.integers(200, 3000))
+    p = E.GenParams(n=n, seed=500 + seed, deg_kind=2, deg_lo=1, deg_hi=min(n - 1, 400), alpha=1.8,
+                    wmax=int(rng.integers(2, 60)))
+    g = E.generate(p)
+    _say(f"graphs seed={seed} n={g.n} m={g.m} maxdeg={g.max_outdeg}")
+    ref = lfp(oracle, to_oracle_game(g))
+    with E.Solver(g.n, g.m) as s:
+        s.load(g)
+        for shape in (dict(group=1, big_deg=8, huge_deg=64), dict(group=4, big_deg=16, huge_deg=32, dense_pct=1),
+                      dict(group=2, batch_rounds=7), dict(group=8, certify_every=-1, dense_pct=101)):
+            for graphs in (-1, 1, 1):
+                s.prepare(E.options(graphs=graphs, **shape))
+                conv, st = s.iterate()
+                assert conv
+                vals = s.values()
+                assert np.array_equal(vals, ref), (shape, graphs, np.count_nonzero(vals != ref))
+
+
+def test_graph_replay_on_phased_game(oracle):
+    """Phased dense rounds (one launch per round over 8 successor slices) and hub kernels,
+    replayed as graphs: the same values as kernel-by-kernel launches, certified exact."""
+    import egsolver_amd as E
+    p = E.GenParams(n=(1 << 20) + 4097, seed=77, deg_kind=2, deg_lo=1, deg_hi=3000, alpha=2.2, wmax=40)
+    g = E.generate(p)
+    _say(f"graphs phased n={g.n} m={g.m}")
+    with E.Solver(g.n, g.m) as s:
+        s.load(g)
+        out = {}
+        for graphs in (-1, 1):
+            s.prepare(E.options(graphs=graphs, slices=8))
+            conv, st = s.iterate()
+            assert conv
+            out[graphs] = s.values()
+        assert np.array_equal(out[-1], out[1])
+        vals = out[1]
+        exact_by_certificate(oracle, g, vals, s)
