@@ -103,7 +103,9 @@ struct Slot {
     std::string err;      // and its message
 };
 
-constexpr int kLargeWorkers = 3;     // default large-route threads (each with its own solver and stream)
+// Default large-route threads, each with its own solver and stream: two, so the server's
+// streams (slices, copies and the two workers) fit the four hardware queues a process gets.
+constexpr int kLargeWorkers = 2;
 constexpr int kLargeCus = 64;        // default compute units a slice leaves to the large route
 // Default route threshold: a workgroup per game pays off while the game's values stay in
 // LDS or close to it; from this many vertices the device-wide solver serves it sooner.

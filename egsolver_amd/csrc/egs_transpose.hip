@@ -12,13 +12,15 @@
 //   scan     the counts transposed to [bucket][tile] (LDS tiles, coalesced) and scanned
 //            once: every (bucket, tile) gets its start, buckets following each other and
 //            within a bucket the tiles in edge order; transposed back for the scatter;
-//   scatter  each tile moves its edges, as (source, destination within the bucket) items,
-//            into its place in each bucket.  The tile is cut into one contiguous run per
-//            wave; a wave's items for one bucket keep their edge order (a lane's rank among
-//            the lanes of the same bucket comes from ballots over the bucket bits), so
-//            every bucket receives its items in edge order;
-//   bucket   one workgroup per bucket counts its destinations in LDS, writes cptr for them
-//            and places each item at its destination's next slot, again in order.
+//   scatter  one workgroup per tile sorts the tile's edges by bucket in LDS (stable: waves
+//            take consecutive runs, lanes are ranked by ballots over the bucket bits) and
+//            writes each bucket's run of (source, destination within the bucket) items with
+//            consecutive lanes on consecutive slots; every bucket receives its items in edge
+//            order.  With more than 2048 buckets one wave per tile scatters item by item;
+//   bucket   one workgroup per bucket counts its destinations in LDS, writes cptr for them,
+//            places each item at its destination's next slot, again in order, in LDS, and
+//            writes the bucket's predecessor lists out in one coalesced pass; a bucket too
+//            large for LDS is placed in HBM directly by a four-wave kernel.
 // The predecessor weights the reference also transposes are never read by its kernels
 // (src/dev_EG_alg.cu:283) and are not built.
 #include "egs_device.h"
@@ -30,6 +32,7 @@ constexpr int kTrTile = kTrTileEdges;  // edges per tile of the count and scatte
 constexpr int kTrWaves = 4;            // 256 threads
 constexpr int kTrMaxBuckets = 4096;    // LDS histograms of the count and scatter kernels
 constexpr int kTrMaxWidth = 4096;      // destinations per bucket (LDS counters of the bucket kernel)
+constexpr int64_t kTrBucketLds = 150 * 1024;   // LDS bytes of the LDS bucket kernel at most
 
 // Lanes of this wave whose `key` (of `bits` significant bits) equals this lane's.
 __device__ __forceinline__ unsigned long long match_lanes(int key, int bits) {
@@ -46,21 +49,36 @@ __device__ __forceinline__ unsigned long long lanes_below() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Source of every edge, written into src[m] (cpred's space).  A lane takes a row of up to
-// 64 edges; longer rows are written by the whole wave, so a hub does not hold one lane.
+// Source of every edge, written into src[m] (cpred's space).  A wave takes 64 rows and
+// writes their edges' sources in edge order, 64 consecutive edges per store.  A chunk of
+// 64 edges inside one row (a hub) takes that row directly; otherwise each lane finds the row
+// of its edge among the 64 row starts the lanes hold (a binary search over shuffles; a row
+// without edges is skipped because the search takes the last row starting at or before
+// the edge).
 __global__ __launch_bounds__(256) void k_tr_src(const int *__restrict__ ptr, int n, int *__restrict__ src) {
     const int lane = threadIdx.x & 63;
-    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x; v0 < n; v0 += (int64_t)gridDim.x * blockDim.x) {
-        const int v = (int)(v0 + threadIdx.x);
-        int s = 0, e = 0;
-        if (v < n) { s = ptr[v]; e = ptr[v + 1]; }
-        const bool small = e - s <= 64;
-        if (small)
-            for (int k = s; k < e; k++) src[k] = v;
-        for (unsigned long long heavy = __ballot(!small); heavy; heavy &= heavy - 1) {
-            const int b = __ffsll(heavy) - 1;
-            const int hv = __shfl(v, b, 64), hs = __shfl(s, b, 64), he = __shfl(e, b, 64);
-            for (int k = hs + lane; k < he; k += 64) src[k] = hv;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t v0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64; v0 < n; v0 += nw * 64) {
+        const int64_t v = v0 + lane;
+        const int start = ptr[v < n ? v : n];                 // rows past n start at m
+        const int lo = __shfl(start, 0, 64);
+        const int hi = ptr[v0 + 64 < n ? v0 + 64 : n];
+        int cur = 0;                                          // a row starting at or before the chunk
+        for (int e0 = lo; e0 < hi;) {
+            const int next = cur < 63 ? __builtin_amdgcn_readlane(start, cur + 1) : hi;
+            if (next >= e0 + 64 && hi >= e0 + 64) {           // whole chunks inside row cur
+                const int stop = min(next, hi);
+                for (; e0 + 64 <= stop; e0 += 64) src[e0 + lane] = (int)v0 + cur;
+                continue;
+            }
+            const int e = e0 + lane;
+            int l = 0;
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1)
+                if (__shfl(start, l + step, 64) <= e) l += step;
+            cur = __builtin_amdgcn_readlane(l, 63);
+            if (e < hi) src[e] = (int)v0 + l;
+            e0 += 64;
         }
     }
 }
@@ -89,52 +107,33 @@ __global__ __launch_bounds__(256) void k_tr_matrix(const int *__restrict__ in, i
         if (c0 + j < C && r0 + tx < R) out[(size_t)(c0 + j) * R + r0 + tx] = t[tx][j];
 }
 
-constexpr int kTrU = 4;   // chunks of 64 whose loads are in flight together
+constexpr int kTrU = 8;   // chunks of 64 whose loads are in flight together
 
-__global__ __launch_bounds__(256) void k_tr_scatter(const int *__restrict__ src, const int2 *__restrict__ edges,
-                                                    int64_t m, int bshift, int nb, int tiles,
-                                                    const int *__restrict__ offs, int2 *__restrict__ items) {
-    extern __shared__ int pos_lds[];               // [kTrWaves][nb]
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int *pos = pos_lds + wave * nb;
+// One wave per tile: the tile's run of each bucket is this wave's alone, so it is written
+// contiguously and in edge order (a lane's rank among the lanes of the same bucket comes
+// from ballots over the bucket bits).  pos[] starts at the run starts the scan gave.
+__global__ __launch_bounds__(64) void k_tr_scatter(const int *__restrict__ src, const int2 *__restrict__ edges,
+                                                   int64_t m, int bshift, int nb, const int *__restrict__ offs,
+                                                   int2 *__restrict__ items) {
+    extern __shared__ int pos[];                   // [nb]
+    const int lane = threadIdx.x;
     const int bits = 32 - __clz(nb - 1 > 0 ? nb - 1 : 1);
     const int64_t e0 = (int64_t)blockIdx.x * kTrTile, e1 = min(m, e0 + kTrTile);
-    const int64_t per = (e1 - e0 + kTrWaves - 1) / kTrWaves;
-    const int64_t w0 = min(e1, e0 + wave * per), w1 = min(e1, w0 + per);
-    for (int i = threadIdx.x; i < kTrWaves * nb; i += blockDim.x) pos_lds[i] = 0;
-    __syncthreads();
-    for (int64_t c0 = w0; c0 < w1; c0 += 64 * kTrU) {
-        int d[kTrU];
-#pragma unroll
-        for (int q = 0; q < kTrU; q++) d[q] = c0 + q * 64 + lane < w1 ? edges[c0 + q * 64 + lane].x : -1;
-#pragma unroll
-        for (int q = 0; q < kTrU; q++)
-            if (d[q] >= 0) atomicAdd(&pos[d[q] >> bshift], 1);
-    }
-    __syncthreads();
-    // each wave's run of a bucket starts after the tile's earlier waves' runs of it
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-        int base = offs[(size_t)blockIdx.x * nb + b];
-        for (int w = 0; w < kTrWaves; w++) {
-            const int c = pos_lds[w * nb + b];
-            pos_lds[w * nb + b] = base;
-            base += c;
-        }
-    }
+    for (int b = lane; b < nb; b += 64) pos[b] = offs[(size_t)blockIdx.x * nb + b];
     __syncthreads();
     const int dmask = (1 << bshift) - 1;
-    for (int64_t c0 = w0; c0 < w1; c0 += 64 * kTrU) {
+    for (int64_t c0 = e0; c0 < e1; c0 += 64 * kTrU) {
         int2 ew[kTrU];
         int sv[kTrU];
 #pragma unroll
         for (int q = 0; q < kTrU; q++) {
             const int64_t e = c0 + q * 64 + lane;
-            ew[q] = e < w1 ? edges[e] : make_int2(0, 0);
-            sv[q] = e < w1 ? src[e] : 0;
+            ew[q] = e < e1 ? edges[e] : make_int2(0, 0);
+            sv[q] = e < e1 ? src[e] : 0;
         }
 #pragma unroll
         for (int q = 0; q < kTrU; q++) {
-            const bool valid = c0 + q * 64 + lane < w1;
+            const bool valid = c0 + q * 64 + lane < e1;
             const int bk = ew[q].x >> bshift;
             const unsigned long long eq = match_lanes(valid ? bk : -1, bits + 1);
             const int base = valid ? pos[bk] : 0;
@@ -144,10 +143,14 @@ __global__ __launch_bounds__(256) void k_tr_scatter(const int *__restrict__ src,
     }
 }
 
-// One workgroup per bucket: cptr of its destinations and their predecessor lists.
+// One workgroup per bucket, for buckets the LDS kernel below does not take (more than
+// `skip_below` items): cptr of its destinations and their predecessor lists, placed in HBM.
+// The bucket's items are split into one contiguous run per wave; each wave counts its run
+// per destination, the counts are scanned (destinations in order, within a destination the
+// waves in order), and each wave places its run in order.
 __global__ __launch_bounds__(256) void k_tr_bucket(const int2 *__restrict__ items, int n, int64_t m, int bshift,
-                                                   int tiles, const int *__restrict__ offs, int *__restrict__ cptr,
-                                                   int *__restrict__ cpred) {
+                                                   int tiles, const int *__restrict__ offs, int skip_below,
+                                                   int *__restrict__ cptr, int *__restrict__ cpred) {
     extern __shared__ int cd[];                   // [kTrWaves][width]
     __shared__ int lds[4];
     const int width = 1 << bshift;
@@ -155,6 +158,7 @@ __global__ __launch_bounds__(256) void k_tr_bucket(const int2 *__restrict__ item
     const int b = blockIdx.x;
     const int64_t i0 = offs[(size_t)b * tiles];
     const int64_t i1 = b + 1 < (int)gridDim.x ? offs[(size_t)(b + 1) * tiles] : m;
+    if (i1 - i0 <= skip_below) return;
     const int64_t per = (i1 - i0 + kTrWaves - 1) / kTrWaves;
     const int64_t w0 = min(i1, i0 + wave * per), w1 = min(i1, w0 + per);
     for (int i = threadIdx.x; i < kTrWaves * width; i += blockDim.x) cd[i] = 0;
@@ -168,7 +172,6 @@ __global__ __launch_bounds__(256) void k_tr_bucket(const int2 *__restrict__ item
             if (d[q] >= 0) atomicAdd(&cd[wave * width + d[q]], 1);
     }
     __syncthreads();
-    // destinations in order: each wave's slots of a destination follow the earlier waves'
     int carry = (int)i0;
     for (int d0 = 0; d0 < width; d0 += blockDim.x) {
         const int d = d0 + threadIdx.x;
@@ -210,6 +213,181 @@ __global__ __launch_bounds__(256) void k_tr_bucket(const int2 *__restrict__ item
     }
 }
 
+// Sixteen-wave kernels with their output staged in LDS.  Each wave loads its run once into
+// registers (a tile's 1,024 edges, a bucket's at most 1/16 of kTrLBucketMax items) and never
+// reads it again: a second pass over HBM or L2 was what these kernels spent their time on
+// when 32 CUs' tiles did not fit one XCD's L2.  Per-wave counters are 16 bits, two to a
+// word, added to with 32-bit LDS atomics on the word and read and rewritten as halves.
+constexpr int kTrLWaves = 16;             // 1024 threads
+constexpr int kTrLMaxBuckets = 2048;      // counters of the tile kernel: 16 x 2048 x 2 bytes
+constexpr int kTrLChunks = kTrTile / (kTrLWaves * 64);   // 16 chunks of 64 edges per wave and tile
+constexpr int kTrLBucketChunks = 22;      // chunks of 64 items per wave in the bucket kernel
+constexpr int kTrLBucketMax = kTrLWaves * 64 * kTrLBucketChunks;   // 22,528 items
+
+__device__ __forceinline__ void lds_count16(unsigned *cw, int h) {
+    atomicAdd(&cw[h >> 1], 1u << ((h & 1) << 4));            // half h: bits 16(h&1).. of word h/2
+}
+
+// Largest b in [0, nb) with start[b] <= i (start nondecreasing, start[0] = 0).
+__device__ __forceinline__ int bucket_of_slot(const int *start, int nb, int i) {
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One workgroup per tile of kTrTile edges: the tile's items sorted by bucket in LDS (stable:
+// waves take consecutive runs, lanes are ranked by ballots), then written out so that
+// consecutive lanes write consecutive slots of one bucket's run: the tile's run of each
+// bucket leaves as whole lines instead of one partial line per item.  An item waits in LDS
+// as 32 bits: (source - the tile's first source) << bshift | destination within the bucket,
+// or the edge's offset in the tile in place of the source difference when the tile's
+// sources span 2^(32 - bshift) or more (only possible with vertices that have no edges).
+__global__ __launch_bounds__(1024) void k_tr_scatter_lds(const int *__restrict__ src, const int2 *__restrict__ edges,
+                                                         int64_t m, int bshift, int nb, const int *__restrict__ offs,
+                                                         int2 *__restrict__ items) {
+    extern __shared__ int ldyn[];
+    unsigned *cw = (unsigned *)ldyn;                                    // [16][nb] halves
+    unsigned short *c16 = (unsigned short *)ldyn;
+    int *lstart = ldyn + kTrLWaves * nb / 2;                            // [nb]: local start of each bucket
+    int *delta = lstart + nb;                                           // [nb]: global run start - local start
+    unsigned *slot = (unsigned *)(delta + nb);                          // [kTrTile]
+    __shared__ int lds[kTrLWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int bits = 32 - __clz(nb - 1 > 0 ? nb - 1 : 1);
+    const int dmask = (1 << bshift) - 1;
+    const int64_t e0 = (int64_t)blockIdx.x * kTrTile, e1 = min(m, e0 + kTrTile);
+    const int cnt = (int)(e1 - e0);
+    const int src0 = src[e0];
+    const bool by_source = (int64_t)src[e1 - 1] - src0 < (1ll << (32 - bshift));
+    const int wbase = wave * kTrLChunks * 64;
+    int d[kTrLChunks], sv[kTrLChunks];
+#pragma unroll
+    for (int q = 0; q < kTrLChunks; q++) {
+        const int i = wbase + q * 64 + lane;
+        d[q] = i < cnt ? edges[e0 + i].x : -1;
+        sv[q] = i < cnt ? src[e0 + i] : 0;
+    }
+    for (int i = threadIdx.x; i < kTrLWaves * nb / 2; i += blockDim.x) cw[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kTrLChunks; q++)
+        if (d[q] >= 0) lds_count16(cw, wave * nb + (d[q] >> bshift));
+    __syncthreads();
+    // local starts: buckets in order, within a bucket the waves in order
+    int carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += blockDim.x) {
+        const int b = b0 + threadIdx.x;
+        int c[kTrLWaves], t = 0;
+#pragma unroll
+        for (int w = 0; w < kTrLWaves; w++) {
+            c[w] = b < nb ? c16[w * nb + b] : 0;
+            t += c[w];
+        }
+        int total;
+        const int ex = egs_block_exclusive_scan(t, lds, total);
+        int base = carry + ex;
+        if (b < nb) {
+            lstart[b] = base;
+            delta[b] = offs[(size_t)blockIdx.x * nb + b] - base;
+#pragma unroll
+            for (int w = 0; w < kTrLWaves; w++) {
+                c16[w * nb + b] = (unsigned short)base;
+                base += c[w];
+            }
+        }
+        carry += total;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kTrLChunks; q++) {
+        const bool valid = d[q] >= 0;
+        const int bk = d[q] >> bshift;
+        const unsigned long long eq = match_lanes(valid ? bk : -1, bits + 1);
+        const int base = valid ? c16[wave * nb + bk] : 0;
+        const unsigned key = (unsigned)(by_source ? sv[q] - src0 : wbase + q * 64 + lane);
+        if (valid) slot[base + __popcll(eq & lanes_below())] = (key << bshift) | (unsigned)(d[q] & dmask);
+        if (valid && (eq & lanes_below()) == 0) c16[wave * nb + bk] = (unsigned short)(base + __popcll(eq));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const unsigned x = slot[i];
+        const int key = (int)(x >> bshift);
+        const int b = bucket_of_slot(lstart, nb, i);
+        items[delta[b] + i] = make_int2(by_source ? src0 + key : src[e0 + key], (int)(x & (unsigned)dmask));
+    }
+}
+
+// One workgroup per bucket of at most `cap` (<= kTrLBucketMax) items: its destinations'
+// counts, cptr, and the predecessor lists placed in order in LDS and written out coalesced
+// (a random 4-byte store per item reached HBM as a partial line, about 8x the bytes).
+__global__ __launch_bounds__(1024) void k_tr_bucket_lds(const int2 *__restrict__ items, int n, int64_t m, int bshift,
+                                                        int tiles, const int *__restrict__ offs, int cap,
+                                                        int *__restrict__ cptr, int *__restrict__ cpred) {
+    extern __shared__ int ldyn[];
+    const int width = 1 << bshift;
+    unsigned *cw = (unsigned *)ldyn;                                    // [16][width] halves
+    unsigned short *c16 = (unsigned short *)ldyn;
+    int *out = ldyn + kTrLWaves * width / 2;                            // [cap]
+    __shared__ int lds[kTrLWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int b = blockIdx.x;
+    const int64_t i0 = offs[(size_t)b * tiles];
+    const int64_t i1 = b + 1 < (int)gridDim.x ? offs[(size_t)(b + 1) * tiles] : m;
+    if (i1 - i0 > cap) return;                     // k_tr_bucket places it
+    const int cnt = (int)(i1 - i0);
+    const int per = (cnt + kTrLWaves - 1) / kTrLWaves;
+    const int w0 = min(cnt, wave * per), w1 = min(cnt, w0 + per);
+    int2 it[kTrLBucketChunks];
+#pragma unroll
+    for (int q = 0; q < kTrLBucketChunks; q++) {
+        const int i = w0 + q * 64 + lane;
+        it[q] = i < w1 ? items[i0 + i] : make_int2(0, -1);
+    }
+    for (int i = threadIdx.x; i < kTrLWaves * width / 2; i += blockDim.x) cw[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kTrLBucketChunks; q++)
+        if (it[q].y >= 0) lds_count16(cw, wave * width + it[q].y);
+    __syncthreads();
+    int carry = 0;
+    for (int d0 = 0; d0 < width; d0 += blockDim.x) {
+        const int d = d0 + threadIdx.x;
+        int c[kTrLWaves], deg = 0;
+#pragma unroll
+        for (int w = 0; w < kTrLWaves; w++) {
+            c[w] = c16[w * width + d];
+            deg += c[w];
+        }
+        int total;
+        const int ex = egs_block_exclusive_scan(deg, lds, total);
+        int base = carry + ex;
+        const int64_t u = ((int64_t)b << bshift) + d;
+        if (u < n) cptr[u] = (int)i0 + base;
+#pragma unroll
+        for (int w = 0; w < kTrLWaves; w++) {
+            c16[w * width + d] = (unsigned short)base;
+            base += c[w];
+        }
+        carry += total;
+    }
+    __syncthreads();
+    if (b == gridDim.x - 1 && threadIdx.x == 0) cptr[n] = (int)m;
+#pragma unroll
+    for (int q = 0; q < kTrLBucketChunks; q++) {
+        const bool valid = it[q].y >= 0;
+        const int d = it[q].y;
+        const unsigned long long eq = match_lanes(d, bshift + 1);
+        const int base = valid ? c16[wave * width + d] : 0;
+        if (valid) out[base + __popcll(eq & lanes_below())] = it[q].x;
+        if (valid && (eq & lanes_below()) == 0) c16[wave * width + d] = (unsigned short)(base + __popcll(eq));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) cpred[i0 + i] = out[i];
+}
+
 // Bucket width 2^bshift: at least 2048 destinations and at most kTrMaxBuckets buckets.
 int transpose_bshift(int n) {
     int s = 11;
@@ -245,11 +423,26 @@ hipError_t launch_transpose_sorted(const DevGame &g, int *counts, int *offs, int
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_tr_matrix, dim3((tiles + 63) / 64, (nb + 63) / 64), dim3(256), 0, st, (const int *)counts,
                        nb, tiles, offs);
-    hipLaunchKernelGGL(k_tr_scatter, dim3(tiles), dim3(256), sizeof(int) * (size_t)kTrWaves * nb, st, (const int *)src,
-                       g.edges, m, bshift, nb, tiles, (const int *)offs, items);
-    const size_t lds = sizeof(int) * (size_t)kTrWaves << bshift;
-    hipLaunchKernelGGL(k_tr_bucket, dim3(nb), dim3(256), lds, st, (const int2 *)items, n, m, bshift, tiles,
-                       (const int *)counts, const_cast<int *>(g.cptr), const_cast<int *>(g.cpred));
+    if (nb <= kTrLMaxBuckets) {
+        const size_t lds = sizeof(int) * ((size_t)kTrLWaves * nb / 2 + 2 * (size_t)nb + kTrTile);
+        hipLaunchKernelGGL(k_tr_scatter_lds, dim3(tiles), dim3(1024), lds, st, (const int *)src, g.edges, m, bshift, nb,
+                           (const int *)offs, items);
+    } else {
+        hipLaunchKernelGGL(k_tr_scatter, dim3(tiles), dim3(64), sizeof(int) * (size_t)nb, st, (const int *)src, g.edges,
+                           m, bshift, nb, (const int *)offs, items);
+    }
+    // buckets of at most `cap` items are placed in LDS (16 x width 16-bit counters and
+    // out[cap]); the rest, or every bucket when the counters do not fit, in HBM
+    const int64_t counters = (int64_t)kTrLWaves * 2 << bshift;          // bytes
+    const int cap = (int)std::min<int64_t>(kTrLBucketMax, (kTrBucketLds - counters) / (int64_t)sizeof(int));
+    if (cap > 0) {
+        const size_t lds = (size_t)counters + sizeof(int) * (size_t)cap;
+        hipLaunchKernelGGL(k_tr_bucket_lds, dim3(nb), dim3(1024), lds, st, (const int2 *)items, n, m, bshift, tiles,
+                           (const int *)counts, cap, const_cast<int *>(g.cptr), const_cast<int *>(g.cpred));
+    }
+    hipLaunchKernelGGL(k_tr_bucket, dim3(nb), dim3(256), sizeof(int) * (size_t)kTrWaves << bshift, st,
+                       (const int2 *)items, n, m, bshift, tiles, (const int *)counts, cap > 0 ? cap : -1,
+                       const_cast<int *>(g.cptr), const_cast<int *>(g.cpred));
     return hipGetLastError();
 }
 

@@ -92,10 +92,11 @@ def test_transpose_matches_oracle(oracle):
     restated in the oracle) entry for entry: cptr, and within each column the sources in
     ascending order.  Shapes: a skewed game, games with several destination buckets and edge
     tiles, a vertex count that is not a multiple of the bucket width, complete digraphs
-    (every vertex a predecessor of every other), a single vertex, a ladder, and the
-    full-size skewed game (n = 2^20, m = 4,962,742)."""
+    (every vertex a predecessor of every other; one bucket too large for the LDS placement),
+    a single vertex, a ladder, the full-size skewed game (n = 2^20, m = 4,962,742) and the
+    full-size uniform game (n = 2^22, m = 2^25)."""
     import egsolver_amd as E
-    games = [E.config_game(3, scale=1 / 256), E.config_game(2, scale=1 / 64), E.config_game(3),
+    games = [E.config_game(3, scale=1 / 256), E.config_game(2, scale=1 / 64), E.config_game(3), E.config_game(2),
              E.generate(E.GenParams(n=9001, seed=5, deg_kind=1, deg_lo=1, deg_hi=40, wmax=9)),
              E.generate(E.GenParams(n=300, seed=6, deg_kind=3, deg_lo=1, deg_hi=1, wmax=5)),
              E.generate(E.GenParams(n=2, seed=7, deg_kind=0, deg_lo=1, deg_hi=1, wmax=5)),
