@@ -305,6 +305,15 @@ def plain_solve_sample(E, games, check_against=None):
     return t
 
 
+def worklist_estimate(games, lifted):
+    nv = sum(g.n for g in games)
+    est = 0.0
+    for g in games:
+        deg = np.diff(np.asarray(g.ptr, np.int64)).astype(np.float64)
+        est += g.m + lifted * (g.n / nv) * float((deg * deg).sum()) / g.n
+    return est
+
+
 def batched(E, args):
     from egsolver_amd.autotune import tune_batch
     t0 = time.perf_counter()
@@ -340,7 +349,13 @@ def batched(E, args):
                        f"W log-uniform 10..1000, every 16th a complete digraph of 200..500), seed 4",
            "vertices": nv, "edges": ne, "solves_per_s": len(games) / wall, "batch_wall_s": wall,
            "batch_device_ms": st["ms"], "sweeps": st["sweeps"], "proofs": st["certificates"],
+           "edges_relaxed": st["edges_relaxed"], "lifted": st["lifted"],
            "edges_relaxed_per_s": st["edges_relaxed"] / (st["ms"] * 1e-3),
+           # a worklist (re-evaluate the predecessors of every vertex that rose) relaxes every
+           # edge once, then, per lift, the out-edges of the risen vertex's predecessors:
+           # sum over games of m + lifts * sum_v deg(v)^2 / n with uniform successors; the
+           # lifts are split over games in proportion to their vertices
+           "worklist_edges_estimate": worklist_estimate(games, st["lifted"]),
            "plain_solves_per_s": plain_rate, "plain_sample_games": len(sub),
            "speedup_vs_plain": (len(games) / wall) / plain_rate, "options": tuned,
            "exact_vs_plain_on_sample": True}

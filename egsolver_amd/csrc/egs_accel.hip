@@ -668,7 +668,7 @@ hipError_t launch_hbest_reset(const DevGame &g, hipStream_t st) {
 __global__ __launch_bounds__(256) void k_mark_top(DevGame g, const long long *__restrict__ h, int *__restrict__ count) {
     int local = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
-        if (h[v] >= 0) { g.val[v] = EGS_TOP; local++; }
+        if (h[v] >= 0) { g.val[v] = EGS_TOP; g.ring[15] = 1; local++; }
     }
     block_add(local, count);
 }

@@ -738,14 +738,12 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     }
     double t0 = 0;
     HIP_TRY(mark_start(s, &t0));
-    HIP_TRY(hipMemsetAsync(s->d_stats, 0, sizeof(unsigned long long) * 4 * egs::kStatSlots, s->stream));
-    HIP_TRY(hipMemsetAsync(s->d_rounds, 0, sizeof(int), s->stream));
+    HIP_TRY(egs::launch_prepare_reset(s->g, s->stream));   // statistics, counters, modes, ring slots
     // The transpose only feeds the sparse round modes, and many solves never need them
     // (config-2-like games sweep densely until a proof ends them).  Rounds stay dense while
     // it is missing; a round that would have gone sparse raises ring[13], and the host
     // builds the transpose at its next probe and opens sparse rounds (ring[12]).
     s->csc_built = false;
-    HIP_TRY(hipMemsetAsync(s->d_ring + 12, 0, 2 * sizeof(int), s->stream));
     if (s->g.dense_min > s->g.n) HIP_TRY(build_csc(s));     // dense sweeps switched off
     if (s->sh.use_block) {           // shared lifts of the vertices above huge_deg start empty
         HIP_TRY(hipMemsetD32Async((hipDeviceptr_t)s->d_hacc, EGS_TOP, (size_t)s->g.n, s->stream));
@@ -755,13 +753,11 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     HIP_TRY(egs::launch_classify(s->g, s->d_ring + 9, s->stream));
     HIP_TRY(egs::launch_slices(s->g, s->stream));
     HIP_TRY(egs::launch_init(s->g, s->stream));
-    HIP_TRY(hipMemsetAsync(s->d_ring + 11, 0, sizeof(int), s->stream));
-    HIP_TRY(hipMemcpyAsync(s->d_ring + 14, s->h_probe + kPinOne, sizeof(int), hipMemcpyHostToDevice, s->stream));
-    float ms = 0.f;
-    HIP_TRY(mark_stop(s, t0, &ms));
+    // the initial frontier's counters ride on the same wait as the timing
     HIP_TRY(hipMemcpyAsync(s->h_probe + kPinInit, s->d_lcnt + 1 * egs::kListCounters,
                            sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
+    float ms = 0.f;
+    HIP_TRY(mark_stop(s, t0, &ms));
     memset(&s->st, 0, sizeof(s->st));
     s->st.ms_prepare = ms;
     s->st.initial_active = (int32_t)list_total(s, s->h_probe + kPinInit);
@@ -773,14 +769,19 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     return EGS_OK;
 }
 
-static int collect_stats(egs_solver *s) {
+// The run's statistics: queued on the stream before the run's closing wait, read after it.
+static int queue_stats(egs_solver *s) {
     HIP_TRY(egs::launch_sum_stats(s->g, egs::kStatSlots, s->d_statsum, s->stream));
     unsigned long long *h = (unsigned long long *)(s->h_probe + kPinStats);
     HIP_TRY(hipMemcpyAsync(h, s->d_statsum, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipMemcpyAsync(s->h_probe + kPinRounds, s->d_rounds, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipMemcpyAsync(s->h_probe + kPinConv, s->d_lcnt + (s->next_round % 3) * egs::kListCounters,
                            sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
+    return EGS_OK;
+}
+
+static int collect_stats(egs_solver *s) {
+    const unsigned long long *h = (const unsigned long long *)(s->h_probe + kPinStats);
     s->st.processed = (int64_t)h[0];
     s->st.edges_relaxed = (int64_t)h[1];
     s->st.lifted = (int64_t)h[2];
@@ -881,9 +882,11 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
         if (s->stop_flag && *s->stop_flag) break;
         if (batch < s->batch_rounds) batch = std::min(s->batch_rounds, batch * 2);
     }
+    int rc = queue_stats(s);
+    if (rc) return rc;
     float ms = 0.f;
-    HIP_TRY(mark_stop(s, t0, &ms));
-    int rc = collect_stats(s);
+    HIP_TRY(mark_stop(s, t0, &ms));                  // also the wait for the statistics
+    rc = collect_stats(s);
     if (rc) return rc;
     s->st.ms_iterate += ms;
     if (st) *st = s->st;
@@ -944,6 +947,8 @@ int egs_solver_set_values(egs_solver *s, const int32_t *host_values) {
     if (!s || !s->prepared || !host_values) return fail(EGS_E_STATE, "egs_solver_set_values: call egs_solver_prepare first");
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipMemcpyAsync(s->d_val, host_values, sizeof(int) * (size_t)s->g.n, hipMemcpyHostToDevice, s->stream));
+    // the values may hold TOP: the phased rounds then read their TOP masks (ring[15])
+    HIP_TRY(hipMemcpyAsync(s->d_ring + 15, s->h_probe + kPinOne, sizeof(int), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return EGS_OK;
 }

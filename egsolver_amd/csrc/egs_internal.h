@@ -37,7 +37,8 @@ struct DevGame {
                           // [11] a round that must sweep all vertices (after a TOP proof),
                           // [12] 1 once the transpose exists (sparse rounds allowed),
                           // [13] rounds that would have gone sparse without it,
-                          // [14] the absolute number of the next batch's first round
+                          // [14] the absolute number of the next batch's first round,
+                          // [15] nonzero once any vertex has been set to TOP
     const int *sbig;      // static: vertices with big_deg < deg <= huge_deg
     const int *shuge;     // static: vertices with deg > huge_deg
     int big_deg, huge_deg;
@@ -132,6 +133,7 @@ int pack_shift(int n, int wmax);
 // Round j of a batch (absolute round ring[14] + j); launch_advance ends the batch.
 hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int j, hipStream_t st);
 hipError_t launch_advance(const DevGame &g, int rounds, hipStream_t st);
+hipError_t launch_prepare_reset(const DevGame &g, hipStream_t st);
 hipError_t launch_sum_stats(const DevGame &g, int nslots, unsigned long long *out, hipStream_t st);
 hipError_t launch_degree_stats(const int *ptr, int n, int *out, hipStream_t st);
 hipError_t launch_violations(const DevGame &g, unsigned long long *out, hipStream_t st);

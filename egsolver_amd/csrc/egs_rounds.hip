@@ -343,7 +343,7 @@ __device__ __forceinline__ void body_dense(const DevGame &g, const Stage &st, in
         if (valid && gl == 0) {
             acc.processed++;
             acc.edges += (uint32_t)(e - s);
-            if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; }
+            if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; if (t == EGS_TOP) g.ring[15] = 1; }
         }
         stage_push(g, st, lifted && gl == 0, v);
         if ((it & 7) == 7) stage_flush(g, st, kLdsCap / 2);
@@ -378,7 +378,7 @@ __device__ __forceinline__ void body_queue(const DevGame &g, const Stage &st, co
         if (valid && gl == 0) {
             acc.processed++;
             acc.edges += (uint32_t)(e - s);
-            if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; }
+            if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; if (t == EGS_TOP) g.ring[15] = 1; }
         }
         push_preds<G>(g, st, v, lifted, gl, r, acc);
         if ((it & 3) == 3) stage_flush(g, st, kLdsCap / 2);
@@ -417,6 +417,7 @@ __device__ __forceinline__ void body_lifted(const DevGame &g, const Stage &st, c
                     acc.edges += (uint32_t)(e - s);
                     if (old < t) {
                         g.val[p] = t;
+                        if (t == EGS_TOP) g.ring[15] = 1;
                         acc.lifted++;
                         if (g.rise) g.rise[p] = r;
                         const int qs = g.cptr[p], qe = g.cptr[p + 1];
@@ -517,12 +518,14 @@ __global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int j) {
     const int64_t W = gridDim.x;
     const int K = blockIdx.x < tiles ? (int)((tiles - blockIdx.x + W - 1) / W) : 0;
     const int wave = threadIdx.x >> 6;
-    // vertices already at TOP (final) skip their gathers; at most 32 tiles per workgroup
+    // vertices already at TOP (final) skip their gathers; at most 32 tiles per workgroup.
+    // ring[15] tells whether any vertex has reached TOP (until then the mask is not read)
     uint32_t settled = 0;
-    for (int k = 0; k < K; k++) {
-        const int64_t v = (blockIdx.x + k * W) * kSliceTile + threadIdx.x;
-        if (v < g.n && g.val[v] == EGS_TOP) settled |= 1u << k;
-    }
+    if (g.ring[15])
+        for (int k = 0; k < K; k++) {
+            const int64_t v = (blockIdx.x + k * W) * kSliceTile + threadIdx.x;
+            if (v < g.n && g.val[v] == EGS_TOP) settled |= 1u << k;
+        }
     uint64_t edges = 0;
     for (int s = 0; s < P; s++) {
         for (int k = 0; k < K; k++) {
@@ -560,6 +563,7 @@ __global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int j) {
             acc.processed++;
             if (old < t) {
                 g.val[v] = t;
+                if (t == EGS_TOP) g.ring[15] = 1;
                 if (g.rise) g.rise[v] = r;
                 acc.lifted++;
                 lifted = true;
@@ -695,7 +699,7 @@ __global__ __launch_bounds__(256) void k_lift_wave(DevGame g, int j) {
         if (lane == 0) {
             acc.processed++;
             acc.edges += (uint32_t)(e - s);
-            if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; }
+            if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; if (t == EGS_TOP) g.ring[15] = 1; }
         }
         if (dense) stage_push(g, st, lifted && lane == 0, v);
         else push_preds<64>(g, st, v, lifted, lane, r, acc);
@@ -776,7 +780,7 @@ __global__ __launch_bounds__(256) void k_lift_block(DevGame g, int j) {
             const bool lifted = old < t;
             if (threadIdx.x == 0) {
                 acc.processed++;
-                if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; }
+                if (lifted) { g.val[v] = t; acc.lifted++; if (g.rise) g.rise[v] = r; if (t == EGS_TOP) g.ring[15] = 1; }
             }
             if (dense) stage_push(g, st, lifted && threadIdx.x == 0, v);
             else push_preds<256>(g, st, v, lifted, (int)threadIdx.x, r, acc);
@@ -792,11 +796,8 @@ __global__ __launch_bounds__(256) void k_lift_block(DevGame g, int j) {
 // host side
 // ---------------------------------------------------------------------------------------
 
+// The counters, modes and ring slots start cleared (k_prepare_reset).
 hipError_t launch_init(const DevGame &g, hipStream_t st) {
-    hipError_t err = hipMemsetAsync(g.lcnt, 0, sizeof(int) * 3 * kListCounters, st);
-    if (err == hipSuccess) err = hipMemsetAsync(g.mode, 0, sizeof(int) * 3, st);   // MODE_QUEUE
-    if (err == hipSuccess) err = hipMemsetAsync(g.ring, 0, sizeof(int) * 9, st);
-    if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_init, dim3(grid_for(g.n, 256, 2048)), dim3(256), 0, st, g);
     hipLaunchKernelGGL(k_init_big, dim3(256), dim3(256), 0, st, g);
     return hipGetLastError();

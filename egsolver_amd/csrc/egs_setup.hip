@@ -217,20 +217,24 @@ __global__ void k_degree_max(const int *__restrict__ ptr, int n, int *__restrict
     if ((threadIdx.x & 63) == 0) atomicMax(out, mx);
 }
 
-__global__ void k_sum_stats(const unsigned long long *__restrict__ slots, int nslots,
-                            unsigned long long *__restrict__ out) {
-    __shared__ unsigned long long part[4][256];
+__global__ __launch_bounds__(1024) void k_sum_stats(const unsigned long long *__restrict__ slots, int nslots,
+                                                    unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long part[4][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     unsigned long long s[4] = {0, 0, 0, 0};
     for (int b = threadIdx.x; b < nslots; b += blockDim.x) {
 #pragma unroll
         for (int k = 0; k < 4; k++) s[k] += slots[(size_t)b * 4 + k];
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++) part[k][threadIdx.x] = s[k];
+    for (int k = 0; k < 4; k++) {
+        for (int d = 32; d >= 1; d >>= 1) s[k] += __shfl_xor(s[k], d, 64);
+        if (lane == 0) part[k][wave] = s[k];
+    }
     __syncthreads();
     if (threadIdx.x < 4) {
         unsigned long long t = 0;
-        for (int i = 0; i < (int)blockDim.x; i++) t += part[threadIdx.x][i];
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[threadIdx.x][w];
         out[threadIdx.x] = t;
     }
 }
@@ -296,9 +300,28 @@ hipError_t launch_transpose(const DevGame &g, int *tile_sum, int *fill, hipStrea
     return hipGetLastError();
 }
 
+// Everything a prepare clears, in one launch (in place of a fill per array): the round
+// statistics, the rounds counter, the list and deferral counters, the round modes (queue)
+// and the ring slots, with ring[14], the number of the first round, set to 1.
+__global__ __launch_bounds__(256) void k_prepare_reset(DevGame g) {
+    const int64_t nstats = 4LL * kStatSlots;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nstats; i += (int64_t)gridDim.x * blockDim.x)
+        g.stats[i] = 0;
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < 3 * kListCounters; i += blockDim.x) g.lcnt[i] = 0;
+        if (threadIdx.x < 3) g.mode[threadIdx.x] = MODE_QUEUE;
+        if (threadIdx.x < 16) g.ring[threadIdx.x] = threadIdx.x == 14 ? 1 : 0;
+        if (threadIdx.x == 0) *g.rounds_done = 0;
+    }
+}
+
+hipError_t launch_prepare_reset(const DevGame &g, hipStream_t st) {
+    hipLaunchKernelGGL(k_prepare_reset, dim3(64), dim3(256), 0, st, g);
+    return hipGetLastError();
+}
+
+// counts[0..1] (ring[9], ring[10]) start at zero (k_prepare_reset).
 hipError_t launch_classify(const DevGame &g, int *counts, hipStream_t st) {
-    hipError_t err = hipMemsetAsync(counts, 0, sizeof(int) * 2, st);
-    if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_classify, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, st, g.ptr, g.n, g.big_deg,
                        g.huge_deg, const_cast<int *>(g.sbig), const_cast<int *>(g.shuge), counts);
     return hipGetLastError();
@@ -319,7 +342,7 @@ hipError_t launch_degree_stats(const int *ptr, int n, int *out, hipStream_t st) 
 }
 
 hipError_t launch_sum_stats(const DevGame &g, int nslots, unsigned long long *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_sum_stats, dim3(1), dim3(256), 0, st, g.stats, nslots, out);
+    hipLaunchKernelGGL(k_sum_stats, dim3(1), dim3(1024), 0, st, g.stats, nslots, out);
     return hipGetLastError();
 }
 
