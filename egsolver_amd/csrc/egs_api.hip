@@ -710,6 +710,12 @@ static int pick_shape(egs_solver *s, const egs_options *o) {
         g.phase_tiles = 96 * 1024 / (egs::kSliceTile * 12);
         g.phase_grid = (int)((tiles + g.phase_tiles - 1) / g.phase_tiles);
     }
+    // one-sweep games lift their static hub lists in the same launch as the low-degree sweep
+    g.round_grid = sh.grid;
+    g.wave_grid = sh.use_wave ? sh.wave_grid : 0;
+    g.block_grid = sh.use_block ? sh.block_grid : 0;
+    g.hub_fused = g.phases == 1 && (sh.use_wave || sh.use_block) &&
+                  g.round_grid + g.wave_grid + g.block_grid <= egs::kStatSlots;     // a statistics slot per workgroup
     return EGS_OK;
 }
 

@@ -63,6 +63,9 @@ struct DevGame {
     const void *sedges;   // uint32 (w << sshift) | succ when sshift > 0, else int2 {succ, w}
     int sshift;
     const unsigned long long *bigmask;   // bit v: vertex v is above big_deg (lifted by the wave/block kernels)
+    int round_grid;       // workgroups of k_round's low-degree part
+    int hub_fused;        // 1: k_round's grid continues with the hub part in dense rounds
+    int wave_grid, block_grid;   // workgroups of the wave and workgroup-per-vertex hub bodies
     int phase_grid;       // workgroups of the phased kernels (about what the device holds at once)
     int phase_tiles;      // tiles of the slice layout per workgroup, at most 32 (96 KB of LDS
                           // at 12 bytes per vertex; the dense kernel keeps a 32-bit mask of them)

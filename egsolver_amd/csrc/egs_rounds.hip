@@ -322,12 +322,14 @@ __global__ __launch_bounds__(256) void k_init_big(DevGame g) {
 // round kernels
 // ---------------------------------------------------------------------------------------
 
+// The bodies of k_round stride over the low-degree part of its grid, g.round_grid workgroups
+// (the grid continues with hub workgroups in fused rounds).
 template <int G>
 __device__ __forceinline__ void body_dense(const DevGame &g, const Stage &st, int r, LiftAcc &acc) {
     constexpr int ITEMS = 256 / G;
     const int gl = threadIdx.x % G;
     int it = 0;
-    for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < g.n; t0 += (int64_t)gridDim.x * ITEMS, it++) {
+    for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < g.n; t0 += (int64_t)g.round_grid * ITEMS, it++) {
         const int v = (int)(t0 + threadIdx.x / G);
         bool valid = v < g.n;
         int s = 0, e = 0, old = 0;
@@ -357,7 +359,7 @@ __device__ __forceinline__ void body_queue(const DevGame &g, const Stage &st, co
     const int gl = threadIdx.x % G;
     const int total = pre[kSub + 1];
     int it = 0;
-    for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < total; t0 += (int64_t)gridDim.x * ITEMS, it++) {
+    for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < total; t0 += (int64_t)g.round_grid * ITEMS, it++) {
         const int64_t item = t0 + threadIdx.x / G;
         bool valid = item < total;
         int v = 0, s = 0, e = 0, old = 0;
@@ -394,7 +396,7 @@ __device__ __forceinline__ void body_lifted(const DevGame &g, const Stage &st, c
     const int gl = threadIdx.x % G;
     const int total = pre[kSub + 1];
     int it = 0;
-    for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < total; t0 += (int64_t)gridDim.x * ITEMS, it++) {
+    for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < total; t0 += (int64_t)g.round_grid * ITEMS, it++) {
         const int64_t item = t0 + threadIdx.x / G;
         int cs = 0, ce = 0;
         if (item < total) {
@@ -436,12 +438,23 @@ __device__ __forceinline__ void body_lifted(const DevGame &g, const Stage &st, c
     }
 }
 
-template <int G>
+__device__ __forceinline__ void hub_wave(const DevGame &g, int r, int mode, int bid, int nblk, int *sbuf, int *scnt);
+__device__ __forceinline__ void hub_block(const DevGame &g, int r, int mode, int bid, int nblk, int *sbuf, int *scnt,
+                                          int *wmin, int *s_key, int *s_last);
+
+// A round.  Workgroups [0, round_grid) sweep or walk the low-degree vertices; with hub_fused
+// (one-sweep games with vertices above big_deg) the workgroups past them lift the static
+// big and huge lists in dense rounds at the same time, [round_grid, + wave_grid) a wave per
+// vertex and the rest split over the huge ones; in sparse rounds those return and the wave
+// and workgroup kernels lift the deferred lists after this kernel.
+template <int G, bool HUBS>
 __global__ __launch_bounds__(256) void k_round(DevGame g, int j) {
     const int r = g.ring[14] + j;
     __shared__ int pre[kSub + 2];
     __shared__ int sbuf[kLdsCap];
     __shared__ int scnt, smode;
+    __shared__ int wmin[4];
+    __shared__ int s_key, s_last;
     const int in_slot = r % 3;
     if (threadIdx.x == 0) scnt = 0;
     load_layout(g, in_slot, pre);
@@ -465,6 +478,13 @@ __global__ __launch_bounds__(256) void k_round(DevGame g, int j) {
         }
     }
     if (mode == MODE_DONE) return;
+    if (HUBS && (int)blockIdx.x >= g.round_grid) {       // hub part of a fused round
+        if (mode != MODE_DENSE) return;
+        const int hb = (int)blockIdx.x - g.round_grid;
+        if (hb < g.wave_grid) hub_wave(g, r, mode, hb, g.wave_grid, sbuf, &scnt);
+        else hub_block(g, r, mode, hb - g.wave_grid, g.block_grid, sbuf, &scnt, wmin, &s_key, &s_last);
+        return;
+    }
     if (mode == MODE_DENSE && g.phases > 1) return;    // the phase kernels sweep this round
     const Stage st{sbuf, &scnt, g.list[(r + 1) & 1], g.lcnt + ((r + 1) % 3) * kListCounters,
                    (int)(blockIdx.x % kSub)};
@@ -670,25 +690,20 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
 }
 
 // One wavefront per vertex: the static big list in dense rounds, the deferred list in
-// sparse rounds.
-__global__ __launch_bounds__(256) void k_lift_wave(DevGame g, int j) {
-    const int r = g.ring[14] + j;
-    __shared__ int sbuf[kLdsCap];
-    __shared__ int scnt;
-    const int mode = g.mode[r % 3];
-    if (mode == MODE_DONE) return;
+// sparse rounds.  Workgroup `bid` of `nblk` (the wave kernel's grid, or the hub part of a
+// round kernel's grid).
+__device__ __forceinline__ void hub_wave(const DevGame &g, int r, int mode, int bid, int nblk, int *sbuf, int *scnt) {
     const bool dense = mode == MODE_DENSE;
     const int *items = dense ? g.sbig : g.dbig;
     const int cnt = dense ? g.ring[9] : g.ring[3 + r % 3];
     if (cnt == 0) return;
-    if (threadIdx.x == 0) scnt = 0;
+    if (threadIdx.x == 0) *scnt = 0;
     __syncthreads();
-    const Stage st{sbuf, &scnt, g.list[(r + 1) & 1], g.lcnt + ((r + 1) % 3) * kListCounters,
-                   (int)(blockIdx.x % kSub)};
+    const Stage st{sbuf, scnt, g.list[(r + 1) & 1], g.lcnt + ((r + 1) % 3) * kListCounters, bid % kSub};
     LiftAcc acc;
     const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t wave = (int64_t)bid * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t waves = (int64_t)nblk * (blockDim.x >> 6);
     for (int64_t i = wave; i < cnt; i += waves) {
         const int v = items[i];
         const int s = g.ptr[v], e = g.ptr[v + 1];
@@ -708,6 +723,16 @@ __global__ __launch_bounds__(256) void k_lift_wave(DevGame g, int j) {
     flush_stats(g, acc);
 }
 
+__global__ __launch_bounds__(256) void k_lift_wave(DevGame g, int j) {
+    const int r = g.ring[14] + j;
+    __shared__ int sbuf[kLdsCap];
+    __shared__ int scnt;
+    const int mode = g.mode[r % 3];
+    if (mode == MODE_DONE) return;
+    if (mode == MODE_DENSE && g.hub_fused) return;     // the round kernel's hub part did it
+    hub_wave(g, r, mode, (int)blockIdx.x, (int)gridDim.x, sbuf, &scnt);
+}
+
 // Vertices above huge_deg: the static huge list in dense rounds, the deferred list in
 // sparse rounds.  A vertex is shared by up to `per` workgroups (the grid over the items):
 // each takes every per-th chunk of kHugeChunk edges, folds its minimum key into hacc[v]
@@ -716,28 +741,21 @@ __global__ __launch_bounds__(256) void k_lift_wave(DevGame g, int j) {
 // walking a hub of 70K edges alone made this kernel most of a skewed game's round.
 constexpr int kHugeChunk = 2048;
 
-__global__ __launch_bounds__(256) void k_lift_block(DevGame g, int j) {
-    const int r = g.ring[14] + j;
-    __shared__ int sbuf[kLdsCap];
-    __shared__ int scnt;
-    __shared__ int wmin[4];
-    __shared__ int s_key, s_last;
-    const int mode = g.mode[r % 3];
-    if (mode == MODE_DONE) return;
+__device__ __forceinline__ void hub_block(const DevGame &g, int r, int mode, int bid, int nblk, int *sbuf, int *scnt,
+                                          int *wmin, int *s_key, int *s_last) {
     const bool dense = mode == MODE_DENSE;
     const int *items = dense ? g.shuge : g.dhuge;
     const int cnt = dense ? g.ring[10] : g.ring[6 + r % 3];
     if (cnt == 0) return;
-    if (threadIdx.x == 0) scnt = 0;
+    if (threadIdx.x == 0) *scnt = 0;
     __syncthreads();
-    const Stage st{sbuf, &scnt, g.list[(r + 1) & 1], g.lcnt + ((r + 1) % 3) * kListCounters,
-                   (int)(blockIdx.x % kSub)};
+    const Stage st{sbuf, scnt, g.list[(r + 1) & 1], g.lcnt + ((r + 1) % 3) * kListCounters, bid % kSub};
     LiftAcc acc;
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
-    const int per = max(1, (int)gridDim.x / cnt);        // workgroups per vertex
+    const int per = max(1, nblk / cnt);                  // workgroups per vertex
     const int64_t units = (int64_t)cnt * per;
-    for (int64_t u = blockIdx.x; u < units; u += gridDim.x) {
+    for (int64_t u = bid; u < units; u += nblk) {
         const int i = (int)(u % cnt), part = (int)(u / cnt);
         const int v = items[i];
         const int s = g.ptr[v], e = g.ptr[v + 1];
@@ -770,12 +788,12 @@ __global__ __launch_bounds__(256) void k_lift_block(DevGame g, int j) {
                     g.hcnt[v] = 0;
                 }
             }
-            s_key = key;
-            s_last = last;
+            *s_key = key;
+            *s_last = last;
         }
         __syncthreads();
-        if (s_last) {
-            const int t = egs_unkey(s_key, owner1);
+        if (*s_last) {
+            const int t = egs_unkey(*s_key, owner1);
             const int old = g.val[v];
             const bool lifted = old < t;
             if (threadIdx.x == 0) {
@@ -790,6 +808,18 @@ __global__ __launch_bounds__(256) void k_lift_block(DevGame g, int j) {
     }
     stage_flush(g, st, 1);
     flush_stats(g, acc);
+}
+
+__global__ __launch_bounds__(256) void k_lift_block(DevGame g, int j) {
+    const int r = g.ring[14] + j;
+    __shared__ int sbuf[kLdsCap];
+    __shared__ int scnt;
+    __shared__ int wmin[4];
+    __shared__ int s_key, s_last;
+    const int mode = g.mode[r % 3];
+    if (mode == MODE_DONE) return;
+    if (mode == MODE_DENSE && g.hub_fused) return;     // the round kernel's hub part did it
+    hub_block(g, r, mode, (int)blockIdx.x, (int)gridDim.x, sbuf, &scnt, wmin, &s_key, &s_last);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -817,15 +847,20 @@ hipError_t launch_advance(const DevGame &g, int rounds, hipStream_t st) {
 }
 
 hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int j, hipStream_t st) {
+    const dim3 grid(g.round_grid + (g.hub_fused ? g.wave_grid + g.block_grid : 0));
+#define EGS_ROUND(GG)                                                                            \
+    if (g.hub_fused) hipLaunchKernelGGL((k_round<GG, true>), grid, dim3(256), 0, st, g, j);     \
+    else hipLaunchKernelGGL((k_round<GG, false>), grid, dim3(256), 0, st, g, j);
     switch (sh.group) {
-    case 1: hipLaunchKernelGGL(k_round<1>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
-    case 2: hipLaunchKernelGGL(k_round<2>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
-    case 4: hipLaunchKernelGGL(k_round<4>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
-    case 8: hipLaunchKernelGGL(k_round<8>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
-    case 16: hipLaunchKernelGGL(k_round<16>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
-    case 32: hipLaunchKernelGGL(k_round<32>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
-    default: hipLaunchKernelGGL(k_round<64>, dim3(sh.grid), dim3(256), 0, st, g, j); break;
+    case 1: EGS_ROUND(1) break;
+    case 2: EGS_ROUND(2) break;
+    case 4: EGS_ROUND(4) break;
+    case 8: EGS_ROUND(8) break;
+    case 16: EGS_ROUND(16) break;
+    case 32: EGS_ROUND(32) break;
+    default: EGS_ROUND(64) break;
     }
+#undef EGS_ROUND
     if (g.phases > 1) {
         const size_t lds = (size_t)g.phase_tiles * kSliceTile * sizeof(int);
         if (g.sshift)

@@ -30,11 +30,44 @@
 
 #include "../../include/egs.h"
 #include "egs_games.h"
+#include "egs_internal.h"
 #include "egs_pack.h"
 
 namespace egs {
-int pack_shift(int n, int wmax);
 int lanes_for(double avg_deg, const egs_batch_options &o);
+
+// Owner count and MG of a game internalised on the device (submit_read_order), per slot.
+struct ReadOrderMeta {
+    int valid, n0, mg, pad;
+};
+
+// The internal int2 edges of a game internalised on the device, moved into its arena
+// chunk in the route's layout: 32-bit (w << shift) | succ when shift > 0, else int2
+// {succ, w} (small route) or successors and weights apart (large route, `split`).
+__global__ __launch_bounds__(256) void k_ro_place_edges(const int2 *__restrict__ in, int64_t m, int shift, int split,
+                                                        void *__restrict__ out, int *__restrict__ out_w) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+        const int2 x = in[e];
+        if (shift) ((uint32_t *)out)[e] = ((uint32_t)x.y << shift) | (uint32_t)x.x;
+        else if (split) { ((int *)out)[e] = x.x; out_w[e] = x.y; }
+        else ((int2 *)out)[e] = x;
+    }
+}
+
+// meta = {1, n0 = owner-0 count (pre0[n]), MG}; MG fits 31 bits (checked on the host).
+__global__ void k_ro_meta(const int *__restrict__ pre0, int n, const unsigned long long *__restrict__ mg,
+                          ReadOrderMeta *__restrict__ meta) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *meta = ReadOrderMeta{1, pre0[n], (int)*mg, 0};
+}
+
+// The slice's descriptors of games internalised on the device get their owner count and MG.
+__global__ void k_ro_patch_descs(GameDesc *__restrict__ descs, const ReadOrderMeta *__restrict__ meta, int slots) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < slots; t += gridDim.x * blockDim.x)
+        if (meta[t].valid) {
+            descs[t].n0 = meta[t].n0;
+            descs[t].mg = meta[t].mg;
+        }
+}
 }  // namespace egs
 
 namespace {
@@ -101,6 +134,14 @@ struct Slot {
     int shift = 0, wmax = 0, maxdeg = 0;   // large route: edge packing (0 = separate arrays), max |w|, max degree
     int rc = 0;           // large route: the solve's return code
     std::string err;      // and its message
+    bool read_order = false;   // internalised on the device: n0/MG in the device meta,
+    size_t ext_at = 0;         // ext_of_int[n] at this arena offset (values go back in read order)
+};
+
+// Arena range of a device-side internalisation, freed once its kernels are done.
+struct Transient {
+    size_t at = 0, bytes = 0;
+    hipEvent_t done = nullptr;
 };
 
 // Default large-route threads, each with its own solver and stream: two, so the server's
@@ -185,6 +226,9 @@ struct egs_server {
     std::vector<LargeWorker> workers;   // one per thread
     std::vector<std::thread> threads;
     int large_workers = kLargeWorkers;
+    egs::ReadOrderMeta *d_meta = nullptr;   // per slot (games submitted in read order)
+    std::vector<Transient> transients;      // internalisation work space still in use
+    std::vector<hipEvent_t> spare_events;
 };
 
 namespace {
@@ -239,6 +283,13 @@ void large_worker_main(egs_server *s, int wi) {
         s->busy++;
         lk.unlock();
         int rc = hipStreamSynchronize(s->copy) == hipSuccess ? EGS_OK : sfail(EGS_E_HIP, "upload wait failed");
+        if (!rc && s->slots[t].read_order) {        // owner count and MG from the device
+            egs::ReadOrderMeta meta{};
+            if (hipMemcpy(&meta, s->d_meta + t, sizeof(meta), hipMemcpyDeviceToHost) != hipSuccess)
+                rc = sfail(EGS_E_HIP, "read-order meta read failed");
+            s->slots[t].n0 = meta.n0;
+            s->slots[t].mg = meta.mg;
+        }
         if (!rc) rc = solve_large(s, wk, t, &o);
         Slot &sl = s->slots[t];
         sl.rc = rc;
@@ -343,6 +394,21 @@ int submit_large(egs_server *s, int ticket, int32_t n, int32_t n0, int64_t m, in
     return ticket;
 }
 
+// Free the internalisation work space whose kernels have finished.
+void reclaim_transients(egs_server *s) {
+    for (size_t i = 0; i < s->transients.size();) {
+        Transient &t = s->transients[i];
+        if (hipEventQuery(t.done) == hipSuccess) {
+            s->alloc.release(t.at, t.bytes);
+            s->spare_events.push_back(t.done);
+            t = s->transients.back();
+            s->transients.pop_back();
+        } else {
+            i++;
+        }
+    }
+}
+
 // Solve one queued large game with a worker's device-wide solver (on the worker's own
 // stream and host thread); its values go to its arena chunk.  Uploads are complete (the
 // caller synchronised the copy stream).
@@ -405,7 +471,7 @@ int egs_server_create(int device, int64_t arena_bytes, int32_t max_games, egs_se
     }
     s->alloc.init((size_t)arena_bytes);
     const size_t small = al256(sizeof(egs::GameDesc) * max_games) + al256(sizeof(egs::GameState) * max_games) +
-                         al256(4 * (size_t)max_games) + 256;
+                         al256(4 * (size_t)max_games) + 256 + al256(sizeof(egs::ReadOrderMeta) * max_games);
     if (hipMalloc(&s->d_small, small) != hipSuccess) {
         (void)hipFree(s->arena);
         delete s;
@@ -416,6 +482,8 @@ int egs_server_create(int device, int64_t arena_bytes, int32_t max_games, egs_se
     s->d_states = (egs::GameState *)(b + al256(sizeof(egs::GameDesc) * max_games));
     s->d_order = (int *)(b + al256(sizeof(egs::GameDesc) * max_games) + al256(sizeof(egs::GameState) * max_games));
     s->d_counter = (int *)((char *)s->d_order + al256(4 * (size_t)max_games));
+    s->d_meta = (egs::ReadOrderMeta *)((char *)s->d_counter + 256);
+    SHIP(hipMemset(s->d_meta, 0, sizeof(egs::ReadOrderMeta) * max_games));
     s->slots.resize(max_games);
     if (hipHostMalloc((void **)&s->descs, sizeof(egs::GameDesc) * max_games, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&s->states, sizeof(egs::GameState) * max_games, hipHostMallocDefault) != hipSuccess ||
@@ -458,6 +526,8 @@ int egs_server_destroy(egs_server *s) {
     if (s->descs) (void)hipHostFree(s->descs);
     if (s->states) (void)hipHostFree(s->states);
     if (s->order) (void)hipHostFree(s->order);
+    for (auto &t : s->transients) (void)hipEventDestroy(t.done);
+    for (auto e : s->spare_events) (void)hipEventDestroy(e);
     if (s->ev_upload) (void)hipEventDestroy(s->ev_upload);
     if (s->ev_slice) (void)hipEventDestroy(s->ev_slice);
     if (s->ev_a) (void)hipEventDestroy(s->ev_a);
@@ -569,6 +639,152 @@ int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t m
     return ticket;
 }
 
+// Admit a game in read order: owner[n] (0 = owner 0), rptr[n+1], rsucc[m] (successors as
+// read indices) and rw[m], as a .mpg file lists them.  The owner partition (owner-0 vertices
+// first, each block in read order), the internal CSR and MG are made on the device by
+// launch_internalise, where the reference's host driver makes them
+// (src/egsolver.c:107-220,253-304); the host only validates the arrays on their way into
+// staging.  Returns a ticket like egs_server_submit; fetch() returns the values in read
+// order.
+int egs_server_submit_read_order(egs_server *s, int32_t n, int64_t m, const uint8_t *owner, const int32_t *rptr,
+                                 const int32_t *rsucc, const int32_t *rw) {
+    if (!s || n < 1 || m < 1 || m >= 0x7fffffffLL || !owner || !rptr || !rsucc || !rw)
+        return sfail(EGS_E_ARG, "egs_server_submit_read_order: bad game");
+    reclaim_transients(s);
+    int ticket = -1;
+    for (int i = 0; i < s->max_games; i++)
+        if (!s->slots[i].used) { ticket = i; break; }
+    if (ticket < 0) return sfail(EGS_E_CAPACITY, "no free game slot");
+    int32_t wm = 0, dm = 0;
+    int64_t bad = -1;
+    switch (egs_pk_check(n, m, rptr, rsucc, rw, &wm, &dm, &bad)) {
+    case EGS_PK_BAD_OFFSETS: return sfail(EGS_E_ARG, "egs_server_submit_read_order: CSR offsets must run 0..m");
+    case EGS_PK_NO_EDGE:
+        return sfail(EGS_E_ARG, "egs_server_submit_read_order: vertex %lld has no out-edge", (long long)bad);
+    case EGS_PK_BAD_SUCC:
+        return sfail(EGS_E_ARG, "egs_server_submit_read_order: successor out of range at edge %lld", (long long)bad);
+    default: break;
+    }
+    // MG <= n * max |w|; only past INT32_MAX can it overflow, and then it is summed here to
+    // refuse the game as the reference does (src/egsolver.c:283-304)
+    if ((int64_t)n * wm > 0x7fffffffLL) {
+        int64_t mg = 0;
+        for (int32_t v = 0; v < n; v++) {
+            int64_t worst = 0;
+            for (int32_t e = rptr[v]; e < rptr[v + 1]; e++) worst = std::max<int64_t>(worst, -(int64_t)rw[e]);
+            mg += worst;
+        }
+        if (mg > 0x7fffffffLL) return sfail(EGS_E_ARG, "Error too high weights --> overflow (MG past INT32_MAX)");
+    }
+    SHIP(hipSetDevice(s->device));
+    const bool large = n >= s->large_n;
+    if (large) {
+        int rc = start_workers(s);
+        if (rc) return rc;
+    }
+    const int shift = egs::pack_shift(n, wm);
+    const size_t ptr_b = 4 * ((size_t)n + 1), n1 = al256(ptr_b);
+    const size_t body = large ? footprint_large(n, m, shift > 0) : footprint(n, m, wm);
+    const size_t bytes = body + al256(4 * (size_t)n);            // + ext_of_int
+    // work space: the read-order arrays, the partition's scans, the internal int2 edges, MG
+    const size_t o_rptr = al256((size_t)n), o_rsucc = o_rptr + n1, o_rw = o_rsucc + al256(4 * (size_t)m);
+    const size_t in_end = o_rw + al256(4 * (size_t)m);
+    const size_t o_flag0 = in_end, o_pre0 = o_flag0 + n1, o_rev = o_pre0 + n1, o_deg = o_rev + n1;
+    const size_t o_tiles = o_deg + n1, o_edges = o_tiles + al256(4 * ((size_t)n / 4096 + 4));
+    const size_t o_mg = o_edges + al256(8 * (size_t)m), tbytes = o_mg + 256;
+    if (!s->alloc.fits(bytes)) return sfail(EGS_E_CAPACITY, "arena full (%zu bytes wanted)", bytes);
+    int sb = 0;
+    int rc = staging_reserve(s, in_end, &sb);
+    if (rc) return rc;
+    char *stg = (char *)s->staging[sb];
+    egs_pk_copy(stg, owner, (size_t)n);
+    egs_pk_copy(stg + o_rptr, rptr, ptr_b);
+    egs_pk_copy(stg + o_rsucc, rsucc, 4 * (size_t)m);
+    egs_pk_copy(stg + o_rw, rw, 4 * (size_t)m);
+    size_t at = 0, tat = 0;
+    if (!s->alloc.alloc(bytes, at)) return sfail(EGS_E_CAPACITY, "arena full (%zu bytes wanted)", bytes);
+    if (!s->alloc.alloc(tbytes, tat)) {
+        s->alloc.release(at, bytes);
+        return sfail(EGS_E_CAPACITY, "arena full (%zu bytes of work space wanted)", tbytes);
+    }
+    char *base = (char *)s->arena + at, *T = (char *)s->arena + tat;
+    int *ext = (int *)(base + body);
+    SHIP(hipMemcpyAsync(T, stg, in_end, hipMemcpyHostToDevice, s->copy));
+    SHIP(hipEventRecord(s->staging_ev[sb], s->copy));
+    egs::ReadOrderBuffers b{};
+    b.owner = (const uint8_t *)T;
+    b.rptr = (const int *)(T + o_rptr);
+    b.rsucc = (const int *)(T + o_rsucc);
+    b.rw = (const int *)(T + o_rw);
+    b.flag0 = (int *)(T + o_flag0);
+    b.pre0 = (int *)(T + o_pre0);
+    b.rev = (int *)(T + o_rev);
+    b.ext_of_int = ext;
+    b.deg = (int *)(T + o_deg);
+    b.scan_tiles = (int *)(T + o_tiles);
+    unsigned long long *d_mg = (unsigned long long *)(T + o_mg);
+    SHIP(egs::launch_internalise(b, n, (int *)base, (int2 *)(T + o_edges), d_mg, s->copy));
+    const size_t o_e = n1, o_w = o_e + al256(4 * (size_t)m);
+    hipLaunchKernelGGL(egs::k_ro_place_edges, dim3(egs::grid_for(m, 256, 4096)), dim3(256), 0, s->copy,
+                       (const int2 *)(T + o_edges), m, shift, large && !shift ? 1 : 0, (void *)(base + o_e),
+                       (int *)(base + o_w));
+    hipLaunchKernelGGL(egs::k_ro_meta, dim3(1), dim3(64), 0, s->copy, (const int *)b.pre0, n,
+                       (const unsigned long long *)d_mg, s->d_meta + ticket);
+    SHIP(hipGetLastError());
+    Transient tr{tat, tbytes, nullptr};
+    if (!s->spare_events.empty()) {
+        tr.done = s->spare_events.back();
+        s->spare_events.pop_back();
+    } else {
+        SHIP(hipEventCreateWithFlags(&tr.done, hipEventDisableTiming));
+    }
+    SHIP(hipEventRecord(tr.done, s->copy));
+    s->transients.push_back(tr);
+    if (!large) SHIP(hipMemsetAsync(s->d_states + ticket, 0, sizeof(egs::GameState), s->copy));
+    SHIP(hipEventRecord(s->ev_upload, s->copy));
+    Slot &sl = s->slots[ticket];
+    sl = Slot{};
+    sl.used = true;
+    sl.large = large;
+    sl.at = at;
+    sl.bytes = bytes;
+    sl.n = n;
+    sl.m = m;
+    sl.shift = shift;
+    sl.wmax = wm;
+    sl.maxdeg = dm;
+    sl.read_order = true;
+    sl.ext_at = at + body;
+    sl.seq = s->admitted++;
+    s->hi_slot = std::max(s->hi_slot, ticket);
+    egs::GameDesc d{};
+    d.ptr = (const int *)base;
+    d.m = m;
+    d.n = n;
+    if (large) {
+        d.values = (int *)(base + (shift ? o_w : o_w + al256(4 * (size_t)m)));
+        s->descs[ticket] = d;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->large_queue.push_back(ticket);
+        }
+        s->cv_work.notify_one();
+        return ticket;
+    }
+    const size_t edge_b = (shift ? 4 : 8) * (size_t)m;
+    d.edges = base + o_e;
+    d.values = (int *)(base + o_e + al256(edge_b));
+    d.rise = (int *)((char *)d.values + al256(4 * (size_t)n));
+    d.h = (int *)((char *)d.rise + al256(4 * (size_t)n));
+    d.state = s->d_states + ticket;
+    d.wmax = wm;
+    d.shift = shift;
+    d.group = 1;                                   // n0 and MG: patched on the device at launch
+    s->descs[ticket] = d;
+    memset(&s->states[ticket], 0, sizeof(egs::GameState));
+    return ticket;
+}
+
 // Start one slice over every resident unfinished game; returns how many are in it.
 int egs_server_launch(egs_server *s, const egs_serve_options *o) {
     if (!s) return sfail(EGS_E_ARG, "null server");
@@ -625,6 +841,11 @@ int egs_server_launch(egs_server *s, const egs_serve_options *o) {
     SHIP(hipMemcpyAsync(s->d_order, s->order, 4 * s->active.size(), hipMemcpyHostToDevice, s->compute));
     SHIP(hipMemsetAsync(s->d_counter, 0, 4, s->compute));
     SHIP(hipStreamWaitEvent(s->compute, s->ev_upload, 0));
+    bool any_ro = false;
+    for (int i : s->active) any_ro |= s->slots[i].read_order;
+    if (any_ro)   // owner counts and MGs of games internalised on the device
+        hipLaunchKernelGGL(egs::k_ro_patch_descs, dim3(std::max(1, (s->hi_slot + 256) / 256)), dim3(256), 0,
+                           s->compute, s->d_descs, (const egs::ReadOrderMeta *)s->d_meta, s->hi_slot + 1);
     SHIP(hipEventRecord(s->ev_a, s->compute));
     hipLaunchKernelGGL(egs::k_games, dim3(grid), dim3(threads), sizeof(int) * (size_t)lds_ints, s->compute,
                        (const egs::GameDesc *)s->d_descs, (const int *)s->d_order, (int)s->active.size(),
@@ -706,12 +927,23 @@ int egs_server_fetch(egs_server *s, int ticket, int32_t *values, int64_t *sweeps
     if (sl.large && sl.rc) {
         const int rc = sl.rc;
         const std::string msg = sl.err;
+        if (sl.read_order) SHIP(hipMemset(s->d_meta + ticket, 0, sizeof(egs::ReadOrderMeta)));
         s->alloc.release(sl.at, sl.bytes);
         sl = Slot{};
         return sfail(rc, "game %d: %s", ticket, msg.c_str());
     }
     SHIP(hipSetDevice(s->device));
-    if (values) SHIP(hipMemcpy(values, s->descs[ticket].values, 4 * (size_t)sl.n, hipMemcpyDeviceToHost));
+    if (values && sl.read_order) {
+        // internal vertex i is read index ext_of_int[i]
+        std::vector<int32_t> v((size_t)sl.n), ext((size_t)sl.n);
+        SHIP(hipMemcpy(v.data(), s->descs[ticket].values, 4 * (size_t)sl.n, hipMemcpyDeviceToHost));
+        SHIP(hipMemcpy(ext.data(), (char *)s->arena + sl.ext_at, 4 * (size_t)sl.n, hipMemcpyDeviceToHost));
+        for (int32_t i = 0; i < sl.n; i++) values[ext[i]] = v[i];
+    } else if (values) {
+        SHIP(hipMemcpy(values, s->descs[ticket].values, 4 * (size_t)sl.n, hipMemcpyDeviceToHost));
+    }
+    if (sl.read_order) SHIP(hipMemset(s->d_meta + ticket, 0, sizeof(egs::ReadOrderMeta)));
+    reclaim_transients(s);
     if (sweeps) *sweeps = sl.large ? sl.sweeps : s->states[ticket].sweeps;
     if (edges) *edges = sl.large ? sl.edges : s->states[ticket].edges;
     s->alloc.release(sl.at, sl.bytes);

@@ -46,6 +46,7 @@ N.register_signatures({
     "egs_server_set_route": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
     "egs_server_submit": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                          _vp, _vp, _vp]),
+    "egs_server_submit_read_order": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp, _vp]),
     "egs_server_launch": (ctypes.c_int, [_vp, ctypes.POINTER(ServeOptions)]),
     "egs_server_wait": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
     "egs_server_last_slice_ms": (ctypes.c_double, [_vp]),
@@ -115,6 +116,23 @@ class Server:
             return None
         _check(rc, "egs_server_submit")
         self.n_of[rc] = g.n
+        return rc
+
+    def submit_read_order(self, owner, rptr, rsucc, rw):
+        """A game in read order (as a .mpg file lists it; successors are read indices): the
+        owner partition, internal CSR and MG are made on the device.  Ticket, or None when
+        the game does not fit right now; fetch() returns its values in read order."""
+        owner = np.ascontiguousarray(owner, np.uint8)
+        rptr = np.ascontiguousarray(rptr, np.int32)
+        rsucc = np.ascontiguousarray(rsucc, np.int32)
+        rw = np.ascontiguousarray(rw, np.int32)
+        n, m = len(owner), len(rsucc)
+        rc = N.egs().egs_server_submit_read_order(self.h, n, m, owner.ctypes.data, rptr.ctypes.data,
+                                                  rsucc.ctypes.data, rw.ctypes.data)
+        if rc == EGS_E_CAPACITY:
+            return None
+        _check(rc, "egs_server_submit_read_order")
+        self.n_of[rc] = n
         return rc
 
     def launch(self, opt=None):

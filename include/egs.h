@@ -317,6 +317,14 @@ int egs_server_free_slots(egs_server *s);
 /* Returns a ticket >= 0, or EGS_E_CAPACITY when the game does not fit right now. */
 int egs_server_submit(egs_server *s, int32_t n, int32_t n0, int64_t m, int32_t mg, const int32_t *ptr,
                       const int32_t *succ, const int32_t *w);
+/* No reference counterpart.  Admit a game in read order, as a .mpg file lists it:
+ * owner[n] (0 = player 0, token MAX), rptr[n+1], rsucc[m] (successors as read indices),
+ * rw[m].  The owner partition, internal CSR and MG are made on the device (the reference's
+ * host driver does them at src/egsolver.c:107-220,253-304); the host only validates the
+ * arrays while staging them.  A ticket like egs_server_submit's; egs_server_fetch returns
+ * its values in read order. */
+int egs_server_submit_read_order(egs_server *s, int32_t n, int64_t m, const uint8_t *owner, const int32_t *rptr,
+                                 const int32_t *rsucc, const int32_t *rw);
 /* Returns the number of games in the slice started (0: nothing resident). */
 int egs_server_launch(egs_server *s, const egs_serve_options *opt);
 /* Returns how many games finished in the slice; their tickets go to done[0..cap). */

@@ -165,3 +165,51 @@ def test_config5_stream_both_routes(oracle):
             s.iterate()
             assert np.array_equal(s.values(), vals[i]), i
     _say(f"config 5: {wall:.2f} s, {slices} slices; sample {unseeded} exact from 0, {seeded} through the certificate")
+
+
+def test_served_read_order_games(oracle):
+    """Games submitted in read order (owners, read-order CSR with read-index successors, as a
+    .mpg file lists them) are partitioned, internalised and given their MG on the device, on
+    both routes and in both edge layouts (packed and, for wide weights on the large route,
+    successors and weights apart); their values come back in read order and equal the
+    oracle's least fixpoint of the host-internalised game, mapped back to read order."""
+    import egsolver_amd as E
+    params = []
+    for i in range(10):
+        p = E.family_params(E.games.FAMILY_SERVE, 21, i)
+        if p.deg_kind != 3:
+            p.n = int(min(p.n, 3000))
+        params.append(p)
+    params.append(E.GenParams(n=3000, seed=41, deg_kind=1, deg_lo=2, deg_hi=6, wmax=600000))   # wide weights
+    params.append(E.GenParams(n=1200, seed=42, deg_kind=1, deg_lo=2, deg_hi=6, wmax=700000))
+    games = []
+    for i, p in enumerate(params):
+        ro = E.games.generate_read_order(p, stream=i)
+        g = E.games.internalise(*ro)
+        ref = np.empty(g.n, np.int32)
+        ref[g.ext_of_int] = lfp(oracle, to_oracle_game(g))
+        games.append((ro, g, ref))
+    cut = 1500
+    _say(f"read-order submits, cut {cut}, sizes {[g.n for _, g, _ in games]}")
+    arena = sum(2 * E.Server.footprint(g) + 24 * g.m + 32 * g.n for _, g, _ in games) + (1 << 22)
+    with E.Server(arena, max_games=32) as srv:
+        srv.set_route(cut, 2)
+        tickets = {}
+        for k, (ro, g, ref) in enumerate(games):
+            t = srv.submit_read_order(*ro)
+            assert t is not None
+            tickets[t] = k
+        got = {}
+        while len(got) < len(tickets):
+            srv.launch(E.serve_options(slice_kedges=64))
+            for t in srv.wait():
+                got[t], _ = srv.fetch(t)
+        for t, k in tickets.items():
+            ro, g, ref = games[k]
+            assert np.array_equal(got[t], ref), (k, g.n, int((got[t] != ref).sum()))
+        # a read-order game with a vertex that has no out-edge is refused before any upload
+        owner, rptr, rsucc, rw = games[0][0]
+        bad_ptr = rptr.copy()
+        bad_ptr[1] = bad_ptr[0]
+        with pytest.raises(RuntimeError):
+            srv.submit_read_order(owner, bad_ptr, rsucc, rw)

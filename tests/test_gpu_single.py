@@ -450,3 +450,21 @@ def test_graph_replay_on_phased_game(oracle):
         assert np.array_equal(out[-1], out[1])
         vals = out[1]
         exact_by_certificate(oracle, g, vals, s)
+
+
+@pytest.mark.parametrize("group,dense_pct", [(8, 0), (2, 1), (16, 101)])
+def test_fused_hub_rounds_many_strides(oracle, group, dense_pct):
+    """A one-sweep game with wavefront and workgroup hubs, large enough that the low-degree
+    part of the round kernel strides over the vertices many times: with the hub workgroups
+    in the same launch (dense rounds) the low-degree workgroups must still cover every
+    vertex, and the deferred hubs of sparse rounds stay with their own kernels.  Exact by
+    the host fixpoint check, the TOP-set certificate and the oracle's EG0 from it."""
+    import egsolver_amd as E
+    g = E.generate(E.GenParams(n=150_000, seed=61, deg_kind=2, deg_lo=1, deg_hi=20_000, alpha=2.0, wmax=40))
+    _say(f"fused hub rounds n {g.n} m {g.m} maxdeg {g.max_outdeg} group {group} dense_pct {dense_pct}")
+    with E.Solver(g.n, g.m) as s:
+        s.load(g)
+        s.prepare(E.options(group=group, big_deg=64, huge_deg=2048, dense_pct=dense_pct))
+        conv, st = s.iterate()
+        assert conv and s.violations() == 0
+        exact_by_certificate(oracle, g, s.values(), s)
