@@ -211,7 +211,7 @@ int egs_solver_top_certificate(egs_solver *s, int32_t max_passes, int64_t *host_
 /* No reference counterpart.  For callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): egs_solve for a
  * game already on the device, with no allocation of any kind.  The solver's workspace is
- * a fixed array in the library's device memory (18 MiB per device that runs it,
+ * a fixed array in the library's device memory (26 MiB per device that runs it,
  * held until the process ends), so games are limited to 65,536 vertices and 1,048,576
  * edges (EGS_E_CAPACITY beyond).  d_ptr[n+1], d_succ[m], d_w[m] and d_values[n] are device
  * arrays of the current device; work runs on `stream` (a hipStream_t; NULL = the null
