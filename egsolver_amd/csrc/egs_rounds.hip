@@ -176,7 +176,12 @@ __device__ __forceinline__ void flush_stats(const DevGame &g, LiftAcc &a) {
     if (threadIdx.x < 4) {
         unsigned long long s = 0;
         for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[threadIdx.x][i];
-        if (s) g.stats[(size_t)blockIdx.x * 4 + threadIdx.x] += s;
+        // a slot per workgroup; grids past kStatSlots workgroups share slots, so they add atomically
+        unsigned long long *slot = g.stats + (size_t)(blockIdx.x % kStatSlots) * 4 + threadIdx.x;
+        if (s) {
+            if (gridDim.x <= kStatSlots) *slot += s;
+            else atomicAdd(slot, s);
+        }
     }
 }
 
