@@ -651,9 +651,13 @@ __global__ __launch_bounds__(256) void k_bf_huge(DevGame g, long long *__restric
     if (threadIdx.x == 0 && removed) atomicAdd(&flags[0], removed);
 }
 
-// Starting values of the accumulators of k_bf_huge: the identity of each owner's fold.
+// Starting state of the vertices above huge_deg, in one pass: the shared lifts' partial keys
+// and chunk counts (TOP and 0), and the accumulators of k_bf_huge (the identity of each
+// owner's fold) with their flags.
 __global__ __launch_bounds__(256) void k_hbest_reset(DevGame g) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
+        g.hacc[v] = EGS_TOP;
+        g.hcnt[v] = 0;
         g.hbest[v] = v >= g.n0 ? LLONG_MAX : LLONG_MIN;
         g.hflag[v] = 0;
     }
@@ -821,7 +825,7 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
             if (hipMemcpyAsync(fl, flags, sizeof(int) * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
             if (hipMemcpyAsync(cnt, cb.bfcnt, sizeof(int) * kCntSlots, hipMemcpyDeviceToHost, st) != hipSuccess)
                 return -1;
-            if (hipStreamSynchronize(st) != hipSuccess) return -1;
+            if (wait_queued(cb.ev, st) != hipSuccess) return -1;
             if (fl[3] == 0) { rejected = true; break; }
             if (fl[1]) diverged = true;
             for (int p = queued - (queued > 1 ? 0 : 1); p < upto && !diverged; p++) {

@@ -151,7 +151,15 @@ struct CertBuffers {
     int *bfcnt;       // device: ring of 64 pass counters
     unsigned *bits;   // device: 3 two-level bitmaps (ceil(n/32) words + a bit-per-word summary)
     int *pinned;      // host pinned: 2 x 64 + 4 + 64 ints
+    hipEvent_t ev = nullptr;   // waits on read-backs go through this event when set (else the stream)
 };
+// Wait for everything queued on `st` so far: through `ev` (recorded, then waited on) when
+// given, else the whole stream.
+inline hipError_t wait_queued(hipEvent_t ev, hipStream_t st) {
+    if (!ev) return hipStreamSynchronize(st);
+    hipError_t e = hipEventRecord(ev, st);
+    return e == hipSuccess ? hipEventSynchronize(ev) : e;
+}
 struct CertResult {
     int threshold;       // last threshold tried
     int tried;           // candidates tried

@@ -48,7 +48,7 @@ typedef struct egs_options {
                                0 = auto (12), > 100 = never sweep */
     int32_t certify_every;  /* rounds between attempts to prove climbing vertices TOP
                                (exact; attempts back off while they fail);
-                               0 = auto (8), < 0 = never */
+                               0 = auto (7), < 0 = never */
     int32_t cert_max_passes;/* ceiling of the longest-path passes per proof candidate
                                (the cap starts at 64 and doubles while candidates run
                                out of passes); 0 = auto (1024) */
