@@ -136,7 +136,10 @@ typedef struct egs_solver egs_solver;
 int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out);
 int egs_solver_destroy(egs_solver *s);
 
-/* Copy a game in (host arrays or device arrays per `mem`).  Leaves it unprepared. */
+/* Copy a game in (host arrays or device arrays per `mem`).  Leaves it unprepared.  Host
+ * arrays are checked (offsets from 0 to m, an out-edge per vertex, successors in range);
+ * device arrays are trusted to satisfy the same rules, as the reference's parser requires
+ * (every vertex lists at least one successor). */
 int egs_solver_load(egs_solver *s, int mem, int32_t n, int32_t n0, int64_t m, int32_t mg,
                     const int32_t *ptr, const int32_t *succ, const int32_t *w);
 
