@@ -383,7 +383,8 @@ def serving(E, args):
         return solver.values()
     # closed-loop capacities (every request present at t = 0) on the first K requests
     K = min(R, args.serve_capacity_requests)
-    done, wall, _ = run_served([Request(rid=i, arrival=0.0, game=games[i]) for i in range(K)], arena, opt)
+    done, wall, _ = run_served([Request(rid=i, arrival=0.0, game=games[i]) for i in range(K)], arena, opt,
+                               large_n=args.serve_large_n)
     cap = K / wall
     P = min(R, args.serve_plain_requests)
     _, pwall = run_plain([Request(rid=i, arrival=0.0, game=games[i]) for i in range(P)], one)
@@ -394,7 +395,7 @@ def serving(E, args):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=R))
     arrivals -= arrivals[0]
     reqs = [Request(rid=i, arrival=float(arrivals[i]), game=games[i]) for i in range(R)]
-    done, wall, slices = run_served(reqs, arena, opt)
+    done, wall, slices = run_served(reqs, arena, opt, large_n=args.serve_large_n)
     served = latency_stats(done)
     served.update({"solves_per_s": R / wall, "wall_s": wall, "slices": slices, "closed_loop_solves_per_s": cap})
     # the same stream, one game per launch (first P requests: at this rate the plain path
@@ -494,6 +495,7 @@ def main():
     ap.add_argument("--serve-plain-requests", type=int, default=300)
     ap.add_argument("--serve-load", type=float, default=0.75)
     ap.add_argument("--serve-arena-gb", type=float, default=24.0)
+    ap.add_argument("--serve-large-n", type=int, default=0, help="route threshold of the server (0 = its default)")
     ap.add_argument("--slice-kedges", type=int, default=0)
     ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--ingest-scale", type=float, default=0.25, help="share of config 2's vertices for the ingest leg")

@@ -150,7 +150,7 @@ constexpr int kLargeWorkers = 2;
 constexpr int kLargeCus = 64;        // default compute units a slice leaves to the large route
 // Default route threshold: a workgroup per game pays off while the game's values stay in
 // LDS or close to it; from this many vertices the device-wide solver serves it sooner.
-constexpr int kLargeN = 32768;
+constexpr int kLargeN = 16384;
 constexpr size_t kStagingInit = 96u << 20;   // largest pinned staging buffer made at creation
 // Large-route solvers are created for at least this many vertices and edges (one creation
 // covers games up to this size; larger ones grow the worker's solver).

@@ -163,7 +163,7 @@ class Server:
 
     def set_route(self, large_n=0, large_workers=0):
         """Games with at least `large_n` vertices go to the device-wide route (0 = default,
-        32768), solved `large_workers` at a time (0 = default, 2)."""
+        16384), solved `large_workers` at a time (0 = default, 2)."""
         _check(N.egs().egs_server_set_route(self.h, int(large_n), int(large_workers)), "egs_server_set_route")
 
 

@@ -314,7 +314,7 @@ int64_t egs_server_free_bytes(egs_server *s);
 /* Games with at least large_n vertices take the large route: wait() solves them with the
  * device-wide single-game solver, large_workers at a time (host threads, each with its
  * own solver and stream), beside the running slice, instead of one workgroup each.
- * large_n 0 = default (32768); large_workers 0 = default (2), at most 16. */
+ * large_n 0 = default (16384); large_workers 0 = default (2), at most 16. */
 int egs_server_set_route(egs_server *s, int32_t large_n, int32_t large_workers);
 int egs_server_free_slots(egs_server *s);
 /* Returns a ticket >= 0, or EGS_E_CAPACITY when the game does not fit right now. */

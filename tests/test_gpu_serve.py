@@ -131,7 +131,7 @@ def test_served_large_route_wide_weights(oracle):
 def test_config5_stream_both_routes(oracle):
     """A seed-5 serving stream at its distribution (n log-uniform 1e3..5e5, out-degree 2..16,
     W log-uniform 10..1000, every 16th a complete digraph): 600 requests with Poisson
-    arrivals through the server at the default route cut-off (32,768 vertices), so both the
+    arrivals through the server at the default route cut-off (16,384 vertices), so both the
     many-games kernel and the device-wide route serve.  Every result passes the host fixpoint
     check; a sample stratified by size over both routes is bit-exact against the oracle (EG0
     from 0 up to 12,000 vertices, EG0 from the certified TOP set on all) and equals the
@@ -143,7 +143,7 @@ def test_config5_stream_both_routes(oracle):
     rng = np.random.default_rng(5)
     arrivals = np.cumsum(rng.exponential(1 / 2000.0, size=len(games)))
     reqs = [E.Request(rid=i, arrival=float(arrivals[i] - arrivals[0]), game=g) for i, g in enumerate(games)]
-    large = sum(g.n >= 32768 for g in games)
+    large = sum(g.n >= 16384 for g in games)
     _say(f"config 5 stream: {len(games)} requests, {large} on the large route, max n {max(g.n for g in games)}")
     assert large > 100 and len(games) - large > 100
     done, wall, slices = E.run_served(reqs, 24 << 30, E.serve_options(), keep_values=True)
@@ -155,7 +155,7 @@ def test_config5_stream_both_routes(oracle):
         assert viol == 0, (i, g.n, first)
     order = np.argsort([g.n for g in games], kind="stable")
     idx = sorted(set(order[np.linspace(0, len(games) - 1, 24).round().astype(int)].tolist()))
-    assert any(games[i].n >= 32768 for i in idx) and any(games[i].n < 32768 for i in idx)
+    assert any(games[i].n >= 16384 for i in idx) and any(games[i].n < 16384 for i in idx)
     nmax, mmax = max(games[i].n for i in idx), max(games[i].m for i in idx)
     with E.Solver(nmax, mmax) as s:
         unseeded, seeded = exact_sample(oracle, games, vals, idx, s)
