@@ -752,7 +752,7 @@ hipError_t certify_prefetch(const DevGame &g, const CertBuffers &cb, int wmax, h
 }
 
 int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int cap, int max_cands, bool has_big,
-                bool hist_ready, const volatile uint32_t *stop, hipStream_t st, CertResult &res) {
+                bool hist_ready, const volatile uint32_t *stop, int pass_hint, hipStream_t st, CertResult &res) {
     constexpr int kBigGrid = 512;           // workgroups of the kernels for vertices above big_deg
     res = CertResult{};
     if (wmax <= 0 || !g.rise) return 0;
@@ -775,9 +775,10 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
     unsigned *bm[3] = {cb.bits, cb.bits + bm_len, cb.bits + 2 * bm_len};
     // Passes are queued in chunks and read back once per chunk; a pass after the converged
     // one returns at once, so a chunk costs little beyond its real passes.  The first
-    // chunk (closure and first pass, then more) is sized for the common case of a few
-    // passes; later chunks are longer.
-    const int first_chunk = 6, chunk = 8;
+    // chunk (closure and first pass, then more) is sized for a few passes, or for what an
+    // earlier attempt of the run needed plus two, so that an attempt usually ends after one
+    // read-back; later chunks are longer.
+    const int first_chunk = std::max(6, std::min(pass_hint + 2, 32)), chunk = 8;
     // lanes per vertex: about two edges each (on the 2^22-vertex game, with two edges in
     // flight per lane, the solve took 4.41 ms with 4 lanes, 4.43 with 2 and 4.68 with 8)
     int G = 1;
@@ -830,7 +831,7 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
             if (fl[1]) diverged = true;
             for (int p = queued - (queued > 1 ? 0 : 1); p < upto && !diverged; p++) {
                 passes = p + 1;
-                if (cnt[p % kCntSlots] == 0) { converged = true; break; }
+                if (cnt[p % kCntSlots] == 0) { converged = true; res.last_passes = passes; break; }
             }
             queued = upto;
         }

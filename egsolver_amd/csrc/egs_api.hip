@@ -196,6 +196,7 @@ struct egs_solver {
     int wmax = 0;
     int cert_every = 32, cert_gap = 32, cert_max_passes = 1024;
     int cert_cap = 64;        // longest-path passes per candidate; doubles while attempts run out of passes
+    int cert_pass_hint = 0;   // passes the run's last converged candidate needed
     int64_t next_cert = 0;
     int2 *d_edges = nullptr;
     unsigned long long *d_stats = nullptr, *d_statsum = nullptr;
@@ -768,6 +769,7 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     s->cert_gap = s->cert_every;
     s->next_cert = s->cert_every;
     s->cert_cap = std::min(kCertFirstCap, s->cert_max_passes);
+    s->cert_pass_hint = 0;
     s->prepared = true;
     return EGS_OK;
 }
@@ -825,8 +827,10 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
             if (s->stop_flag && *s->stop_flag) break;
             egs::CertResult cr;
             const int got = egs::certify_top(s->g, cb, s->wmax, (int)(s->next_round - 1), s->cert_cap,
-                                             kCertCandidates, s->sh.use_wave != 0, hist, s->stop_flag, s->stream, cr);
+                                             kCertCandidates, s->sh.use_wave != 0, hist, s->stop_flag,
+                                             s->cert_pass_hint, s->stream, cr);
             if (got < 0) return fail(EGS_E_HIP, "certification: %s", hipGetErrorString(hipGetLastError()));
+            if (cr.last_passes > 0) s->cert_pass_hint = cr.last_passes;
             s->st.cert_attempts++;
             s->st.cert_passes += cr.passes;
             s->st.cert_pruned += cr.closure_failed;

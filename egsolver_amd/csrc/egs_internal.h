@@ -167,15 +167,17 @@ struct CertResult {
     int marked;          // vertices set to TOP
     int closure_failed;  // candidates that lost vertices to the closure check
     int cap_hit;         // some candidate ran out of passes (neither converged nor diverged)
+    int last_passes;     // passes of the last candidate that converged (0: none did)
 };
 hipError_t launch_abs_wmax(const DevGame &g, int *out, hipStream_t st);
 int gap_candidates(const int *win, const int *flo, int wmax, int mg, int64_t *out, int max_out);
 // The value histogram of the next attempt, queued with its read-back into the pinned area.
 hipError_t certify_prefetch(const DevGame &g, const CertBuffers &cb, int wmax, hipStream_t st);
 // stop (may be null): polled before each candidate and each chunk of passes; a set flag
-// abandons the attempt with nothing marked.
+// abandons the attempt with nothing marked.  pass_hint: passes an earlier attempt of the
+// same run needed (0: none yet), which sizes the first chunk queued before a read-back.
 int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int cap, int max_cands, bool has_big,
-                bool hist_ready, const volatile uint32_t *stop, hipStream_t st, CertResult &res);
+                bool hist_ready, const volatile uint32_t *stop, int pass_hint, hipStream_t st, CertResult &res);
 // Potentials certifying that every TOP vertex of the current values is won by player 1
 // (cb.h: >= 0 on the certified set, -1 elsewhere).  Returns 1 when the whole TOP set is
 // certified, 0 when not (closure failed, divergence or the pass cap; see res), -1 on a HIP
