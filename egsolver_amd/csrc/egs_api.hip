@@ -822,7 +822,23 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
             egs::CertBuffers cb{s->d_hist, s->d_h, s->d_bfcnt, s->d_bits, s->h_probe + kPinCert, s->ev_b};
             const bool hist = s->wmax > 0;
             if (hist) HIP_TRY(egs::certify_prefetch(s->g, cb, s->wmax, s->stream));
-            HIP_TRY(egs::wait_queued(cb.ev, s->stream));
+            // from the run's second attempt on, one more round runs while the host waits for
+            // these read-backs (a no-op when the frontier was already empty) and the attempt
+            // starts after it; any candidate set is proved or refused exactly, so the
+            // histogram may be a round old.  Not before the first attempt: its proof usually
+            // marks many vertices TOP, after which the same round costs less.
+            const bool ahead = s->st.cert_attempts > 0 && launched < limit && !(s->stop_flag && *s->stop_flag) &&
+                               s->ev_b;
+            if (ahead) {
+                HIP_TRY(hipEventRecord(s->ev_b, s->stream));
+                HIP_TRY(egs::launch_round(s->g, s->sh, 0, s->stream));
+                HIP_TRY(egs::launch_advance(s->g, 1, s->stream));
+                s->next_round++;
+                launched++;
+                HIP_TRY(hipEventSynchronize(s->ev_b));
+            } else {
+                HIP_TRY(egs::wait_queued(cb.ev, s->stream));
+            }
             if (list_total(s, s->h_probe + kPinConv) == 0) break;
             if (s->stop_flag && *s->stop_flag) break;
             egs::CertResult cr;
