@@ -49,46 +49,110 @@ __device__ __forceinline__ unsigned long long lanes_below() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Source of every edge, written into src[m] (cpred's space).  A wave takes 64 rows and
-// writes their edges' sources in edge order, 64 consecutive edges per store.  A chunk of
-// 64 edges inside one row (a hub) takes that row directly; otherwise each lane finds the row
-// of its edge among the 64 row starts the lanes hold (a binary search over shuffles; a row
-// without edges is skipped because the search takes the last row starting at or before
-// the edge).
-__global__ __launch_bounds__(256) void k_tr_src(const int *__restrict__ ptr, int n, int *__restrict__ src) {
+// Rows longer than kTrLong edges are written by the count kernel, a tile's share each, so
+// that no wave walks a hub's edge list alone (a 70,000-edge row took one wave 1,091 serial
+// stores).  The source kernel records, for every kTrLong-edge chunk start, the row that
+// holds it: a long row contains at least one chunk start in every tile it touches, so a
+// tile finds all of its long rows among the rows of its own chunk starts and the next
+// tile's first one.
+constexpr int kTrLongShift = 12;
+constexpr int kTrLong = 1 << kTrLongShift;
+constexpr int kTrTileChunks = kTrTileEdges >> kTrLongShift;      // chunk starts per tile
+static_assert(kTrTileChunks * kTrLong == kTrTileEdges, "tiles hold whole chunks");
+
+// Source of every edge, written into src[m] (cpred's space), except the edges of rows
+// longer than kTrLong.  A wave takes 64 rows and writes their edges' sources in edge order,
+// 64 consecutive edges per store.  A chunk of 64 edges inside one row takes that row
+// directly; otherwise each lane finds the row of its edge among the 64 row starts the lanes
+// hold (a binary search over shuffles; a row without edges is skipped because the search
+// takes the last row starting at or before the edge).  rowof[c] gets the row that holds
+// edge c * kTrLong.
+__device__ __forceinline__ int row_in_wave(int start, int64_t e) {
+    int l = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1)
+        if (__shfl(start, l + step, 64) <= e) l += step;
+    return l;
+}
+
+__global__ __launch_bounds__(256) void k_tr_src(const int *__restrict__ ptr, int n, int *__restrict__ src,
+                                                int *__restrict__ rowof) {
     const int lane = threadIdx.x & 63;
-    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t v0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64; v0 < n; v0 += nw * 64) {
-        const int64_t v = v0 + lane;
-        const int start = ptr[v < n ? v : n];                 // rows past n start at m
-        const int lo = __shfl(start, 0, 64);
-        const int hi = ptr[v0 + 64 < n ? v0 + 64 : n];
+    const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * 64;
+    int64_t v0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
+    // the next 64 rows' starts are loaded while this wave writes the current ones
+    int start_n = 0, hi_n = 0;
+    if (v0 < n) {
+        start_n = ptr[min(v0 + lane, (int64_t)n)];          // rows past n start at m
+        hi_n = ptr[min(v0 + 64, (int64_t)n)];
+    }
+    for (; v0 < n; v0 += stride) {
+        const int start = start_n;
+        const int lo = __builtin_amdgcn_readlane(start, 0);
+        const int hi = __builtin_amdgcn_readfirstlane(hi_n);
+        if (v0 + stride < n) {
+            start_n = ptr[min(v0 + stride + lane, (int64_t)n)];
+            hi_n = ptr[min(v0 + stride + 64, (int64_t)n)];
+        }
+        for (int64_t c0 = ((int64_t)lo + kTrLong - 1) >> kTrLongShift; (c0 << kTrLongShift) < hi; c0 += 64) {
+            const int64_t e = (c0 + lane) << kTrLongShift;
+            const int l = row_in_wave(start, e);
+            if (e < hi) rowof[c0 + lane] = (int)v0 + l;
+        }
+        const int after = __shfl(start, min(lane + 1, 63), 64);
+        const unsigned long long longrows = __ballot((lane < 63 ? after : hi) - start > kTrLong);
         int cur = 0;                                          // a row starting at or before the chunk
         for (int e0 = lo; e0 < hi;) {
             const int next = cur < 63 ? __builtin_amdgcn_readlane(start, cur + 1) : hi;
+            if ((longrows >> cur) & 1) {                      // the count kernel writes it
+                e0 = next;
+                cur++;
+                continue;
+            }
             if (next >= e0 + 64 && hi >= e0 + 64) {           // whole chunks inside row cur
                 const int stop = min(next, hi);
                 for (; e0 + 64 <= stop; e0 += 64) src[e0 + lane] = (int)v0 + cur;
                 continue;
             }
             const int e = e0 + lane;
-            int l = 0;
-#pragma unroll
-            for (int step = 32; step >= 1; step >>= 1)
-                if (__shfl(start, l + step, 64) <= e) l += step;
+            const int l = row_in_wave(start, e);
             cur = __builtin_amdgcn_readlane(l, 63);
-            if (e < hi) src[e] = (int)v0 + l;
+            if (e < hi && !((longrows >> l) & 1)) src[e] = (int)v0 + l;
             e0 += 64;
         }
     }
 }
 
+// Per tile: the histogram of the edges' destination buckets, and the sources of the tile's
+// edges in rows longer than kTrLong (see k_tr_src).
 __global__ __launch_bounds__(256) void k_tr_count(const int2 *__restrict__ edges, int64_t m, int bshift, int nb,
-                                                  int tiles, int *__restrict__ counts) {
+                                                  int tiles, int *__restrict__ counts, const int *__restrict__ ptr,
+                                                  const int *__restrict__ rowof, int *__restrict__ src) {
     __shared__ int hist[kTrMaxBuckets];
+    __shared__ int lrow[kTrTileChunks + 1], lbeg[kTrTileChunks + 1], lend[kTrTileChunks + 1];
+    const int64_t e0 = (int64_t)blockIdx.x * kTrTile, e1 = min(m, e0 + kTrTile);
+    const int64_t nc = (m + kTrLong - 1) >> kTrLongShift;
+    if (threadIdx.x <= kTrTileChunks) {
+        const int64_t c = (int64_t)blockIdx.x * kTrTileChunks + threadIdx.x;
+        int r = -1, b = 0, en = 0;
+        if (c < nc) {
+            r = rowof[c];
+            b = ptr[r];
+            en = ptr[r + 1];
+            if (en - b <= kTrLong) r = -1;
+        }
+        lrow[threadIdx.x] = r;
+        lbeg[threadIdx.x] = b;
+        lend[threadIdx.x] = en;
+    }
     for (int i = threadIdx.x; i < nb; i += blockDim.x) hist[i] = 0;
     __syncthreads();
-    const int64_t e0 = (int64_t)blockIdx.x * kTrTile, e1 = min(m, e0 + kTrTile);
+    for (int k = 0; k <= kTrTileChunks; k++) {
+        const int r = lrow[k];
+        if (r < 0 || (k > 0 && lrow[k - 1] == r)) continue;     // chunk starts of one row are adjacent
+        const int64_t a = max((int64_t)lbeg[k], e0), b = min((int64_t)lend[k], e1);
+        for (int64_t e = a + threadIdx.x; e < b; e += blockDim.x) src[e] = r;
+    }
     for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) atomicAdd(&hist[edges[e].x >> bshift], 1);
     __syncthreads();
     for (int i = threadIdx.x; i < nb; i += blockDim.x) counts[(size_t)blockIdx.x * nb + i] = hist[i];
@@ -413,10 +477,14 @@ hipError_t launch_transpose_sorted(const DevGame &g, int *counts, int *offs, int
     const int nb = (int)(((int64_t)n + (1LL << bshift) - 1) >> bshift);
     const int tiles = (int)((m + kTrTile - 1) / kTrTile);
     int *src = const_cast<int *>(g.cpred);        // sources until k_tr_bucket writes cpred
-    hipLaunchKernelGGL(k_tr_src, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, g.ptr, n, src);
+    // the chunk starts' rows sit in the items' space until the scatter writes it (ceil(m /
+    // kTrLong) ints of its 2m)
+    int *rowof = (int *)items;
+    hipLaunchKernelGGL(k_tr_src, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, g.ptr, n, src, rowof);
     // counts[tile][bucket] -> offs as [bucket][tile] -> scanned into counts ([bucket][tile])
     // -> offs as [tile][bucket], the order the scatter reads
-    hipLaunchKernelGGL(k_tr_count, dim3(tiles), dim3(256), 0, st, g.edges, m, bshift, nb, tiles, counts);
+    hipLaunchKernelGGL(k_tr_count, dim3(tiles), dim3(256), 0, st, g.edges, m, bshift, nb, tiles, counts, g.ptr,
+                       (const int *)rowof, src);
     hipLaunchKernelGGL(k_tr_matrix, dim3((nb + 63) / 64, (tiles + 63) / 64), dim3(256), 0, st, (const int *)counts,
                        tiles, nb, offs);
     hipError_t e = launch_exclusive_scan(offs, nb * tiles, scan_tiles, counts, st);

@@ -87,20 +87,41 @@ def test_random_small_all_paths(oracle, seed):
             assert np.array_equal(vals, ref), (group, big, huge, dense_pct, np.count_nonzero(vals != ref))
 
 
+def _long_rows_game(n=50000, seed=9):
+    """Short rows around rows longer than 4,096 edges: adjacent long rows, rows of exactly
+    4,096 and 4,097 edges, long rows across 4,096-edge chunk and 16,384-edge tile
+    boundaries, one spanning several tiles, and a long last row."""
+    import egsolver_amd as E
+    rng = np.random.default_rng(seed)
+    deg = rng.integers(1, 9, size=n)
+    for i, d in [(100, 4096), (101, 4097), (102, 16384), (103, 16385), (104, 3), (105, 4097),
+                 (5000, 40000), (5001, 4097), (20000, 70000), (n - 1, 5000)]:
+        deg[i] = d
+    ptr = np.zeros(n + 1, np.int32)
+    ptr[1:] = np.cumsum(deg)
+    m = int(ptr[-1])
+    succ = rng.integers(0, n, size=m).astype(np.int32)
+    w = rng.integers(-9, 10, size=m).astype(np.int32)
+    return E.Game(n=n, n0=n // 2, ptr=ptr, succ=succ, w=w, mg=E.games.compute_mg(n, ptr, w),
+                  ext_of_int=np.arange(n, dtype=np.int32))
+
+
 def test_transpose_matches_oracle(oracle):
     """The device transpose equals the reference's host transpose (src/host_csr2csc.c:15-46,
     restated in the oracle) entry for entry: cptr, and within each column the sources in
     ascending order.  Shapes: a skewed game, games with several destination buckets and edge
     tiles, a vertex count that is not a multiple of the bucket width, complete digraphs
     (every vertex a predecessor of every other; one bucket too large for the LDS placement),
-    a single vertex, a ladder, the full-size skewed game (n = 2^20, m = 4,962,742) and the
-    full-size uniform game (n = 2^22, m = 2^25)."""
+    a single vertex, a ladder, the full-size skewed game (n = 2^20, m = 4,962,742), the
+    full-size uniform game (n = 2^22, m = 2^25) and rows longer than 4,096 edges at chunk and
+    tile boundaries (their sources come from the per-tile kernel)."""
     import egsolver_amd as E
     games = [E.config_game(3, scale=1 / 256), E.config_game(2, scale=1 / 64), E.config_game(3), E.config_game(2),
              E.generate(E.GenParams(n=9001, seed=5, deg_kind=1, deg_lo=1, deg_hi=40, wmax=9)),
              E.generate(E.GenParams(n=300, seed=6, deg_kind=3, deg_lo=1, deg_hi=1, wmax=5)),
              E.generate(E.GenParams(n=2, seed=7, deg_kind=0, deg_lo=1, deg_hi=1, wmax=5)),
-             E.generate(E.GenParams(n=70000, seed=8, deg_kind=2, deg_lo=1, deg_hi=20000, alpha=1.6, wmax=50))]
+             E.generate(E.GenParams(n=70000, seed=8, deg_kind=2, deg_lo=1, deg_hi=20000, alpha=1.6, wmax=50)),
+             _long_rows_game()]
     for g in games:
         _say(f"transpose n={g.n} m={g.m}")
         cptr_ref, cpred_ref = oracle.csr2csc(to_oracle_game(g))
