@@ -215,16 +215,7 @@ __global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, lon
 // does what k_closure does after its edge loop, for every vertex.
 constexpr int kClosurePhaseU = 2;   // edges in flight per thread
 
-__device__ __forceinline__ int wave_excl_scan_a(int x) {
-    const int lane = threadIdx.x & 63;
-    int inc = x;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += y;
-    }
-    return inc - x;
-}
+__device__ __forceinline__ int wave_excl_scan_a(int x) { return egs_wave_incl_scan(x) - x; }
 
 template <bool PACK>
 __global__ __launch_bounds__(256) void k_closure_fused(DevGame g, int B, int last, long long bound,

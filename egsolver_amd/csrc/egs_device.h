@@ -56,19 +56,33 @@ __device__ __forceinline__ int egs_wave_sum(int x) {
     return __builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 32);
 }
 
-// Exclusive scan over a workgroup (a multiple of 64 threads): wave scans with shuffles,
-// wave totals combined through lds_warp (one int per wave).  Returns this thread's
-// exclusive prefix; total = the sum over the workgroup.  Every thread must call it.
+// Inclusive prefix sum over the 64 lanes of a wave, every lane active: six DPP adds (VALU
+// only, no LDS round trips).  Shifts by 1, 2, 4 and 8 lanes within each row of 16 (lanes
+// shifted in from below the row read 0), then the rows' totals: lane 15's sum into rows 1
+// and 3 (row_bcast:15), lane 31's into rows 2 and 3 (row_bcast:31).  Rows outside the
+// row mask add the 0 passed as the old value.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int egs_dpp_src(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWMASK, 0xf, false);
+}
+
+__device__ __forceinline__ int egs_wave_incl_scan(int x) {
+    x += egs_dpp_src<0x111, 0xf>(x);   // row_shr:1
+    x += egs_dpp_src<0x112, 0xf>(x);   // row_shr:2
+    x += egs_dpp_src<0x114, 0xf>(x);   // row_shr:4
+    x += egs_dpp_src<0x118, 0xf>(x);   // row_shr:8
+    x += egs_dpp_src<0x142, 0xa>(x);   // row_bcast:15
+    x += egs_dpp_src<0x143, 0xc>(x);   // row_bcast:31
+    return x;
+}
+
+// Exclusive scan over a workgroup (a multiple of 64 threads): wave scans, wave totals
+// combined through lds_warp (one int per wave).  Returns this thread's exclusive prefix;
+// total = the sum over the workgroup.  Every thread must call it.
 __device__ __forceinline__ int egs_block_exclusive_scan(int x, int *lds_warp, int &total) {
-    // wave-level inclusive scan with shuffles, then wave totals combined through LDS.
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
-    int inc = x;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int y = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += y;
-    }
+    const int inc = egs_wave_incl_scan(x);
     if (lane == 63) lds_warp[wid] = inc;
     __syncthreads();
     int wave_off = 0;

@@ -107,12 +107,7 @@ __device__ __forceinline__ void load_layout(const DevGame &g, int slot, int *pre
         int x = 0;
         if (lane < kSub) x = min(c[lane], g.sub_cap);
         int ovf = c[kSub];
-        int inc = x;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int y = __shfl_up(inc, d, 64);
-            if (lane >= d) inc += y;
-        }
+        const int inc = egs_wave_incl_scan(x);
         pre[lane] = inc - x;
         if (lane == 63) {
             pre[kSub] = inc;
@@ -501,17 +496,8 @@ __global__ __launch_bounds__(256) void k_round(DevGame g, int j) {
     flush_stats(g, acc);
 }
 
-// Exclusive prefix sum over the 64 lanes of a wave.
-__device__ __forceinline__ int wave_excl_scan(int x) {
-    const int lane = threadIdx.x & 63;
-    int inc = x;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += y;
-    }
-    return inc - x;
-}
+// Exclusive prefix sum over the 64 lanes of a wave (every lane active).
+__device__ __forceinline__ int wave_excl_scan(int x) { return egs_wave_incl_scan(x) - x; }
 
 // A phased dense round in one launch.  Workgroup b takes the tiles b, b + W, b + 2W, ...
 // (W = the grid) and walks the phases s = 0 .. phases-1 in order, folding each vertex's
@@ -602,66 +588,90 @@ __global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int j) {
     flush_stats(g, acc);
 }
 
-// Edges read ahead per thread in the slice set-up kernel (one thread per vertex).
-constexpr int kSliceU = 8;
-
 // Slice layout of the low-degree vertices' edges in one pass, one tile of kSliceTile
-// vertices per workgroup iteration and no global scan: each thread counts its vertex's
-// edges per slice (the lengths), the workgroup sums them per (slice, wave) and scans the
-// sums in that order from ptr of the tile's first vertex (the wave bases), and each thread
-// then places its edges (read again, mostly from L2) after those of the lower lanes of its
-// wave.  Vertices past n and those above big_deg get empty segments.  The same pass marks
-// the vertices above big_deg in bigmask (a wave ballot) and decides k_init's initial
-// activity of the others into stamp.
+// vertices per workgroup iteration and no global scan.  G lanes read a vertex's edges (a
+// wave reads 64 consecutive edges of 64 / G consecutive vertices per load) and count them
+// per slice with LDS atomics; the workgroup sums the counts per (slice, wave) and scans the
+// sums in that order from ptr of the tile's first vertex (the wave bases); the same lanes
+// then place the edges, each vertex's after those of the lower vertices of its wave.  A
+// vertex's first G edges stay in registers between the two steps, so a game of average
+// degree up to G reads its edges once.  Vertices past n and those above big_deg get empty
+// segments.  The same pass marks the vertices above big_deg in bigmask (a wave ballot) and
+// decides k_init's initial activity of the others into stamp.
+constexpr int kSliceQ = kSliceTile + 8;   // counter row per slice: rows 8 banks apart
+
+template <int G>
 __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *__restrict__ slen,
                                                            int *__restrict__ sbase, void *__restrict__ sedges,
                                                            unsigned long long *__restrict__ bigmask) {
-    // per thread, one column of counters indexed by slice (bank = thread: no conflicts):
-    // the vertex's edge counts, then the next position of its edges in each slice
-    __shared__ int s_cnt[kMaxPhases * kSliceTile];
+    constexpr int kGroups = kSliceTile / G;     // vertices one step of the workgroup covers
+    __shared__ int s_cnt[kMaxPhases * kSliceQ];  // [slice][vertex of the tile]
+    __shared__ int s_beg[kSliceTile], s_end[kSliceTile];
     __shared__ int s_wt[kMaxPhases * kSliceWaves];
     const int P = g.phases;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int *col = s_cnt + threadIdx.x;
+    const int sub = threadIdx.x & (G - 1), grp = threadIdx.x / G;
     const int64_t tiles = ((int64_t)g.n + kSliceTile - 1) / kSliceTile;
     for (int64_t T = blockIdx.x; T < tiles; T += gridDim.x) {
         const int64_t first = T * kSliceTile;
         const int64_t v = first + threadIdx.x;
         const bool valid = v < g.n;
-        int s = 0, e = 0;
+        int vs = 0, ve = 0;
         if (valid) {
-            s = g.ptr[v];
-            e = g.ptr[v + 1];
+            vs = g.ptr[v];
+            ve = g.ptr[v + 1];
         }
-        const bool big = valid && e - s > g.big_deg;
-        const bool mine = valid && !big;
-        for (int q = 0; q < P; q++) col[q * kSliceTile] = 0;
-        if (mine) {
-            bool all_neg = true, any_neg = false;
-            for (int k0 = s; k0 < e; k0 += kSliceU) {
-                int2 ew[kSliceU];
-#pragma unroll
-                for (int j = 0; j < kSliceU; j++) ew[j] = k0 + j < e ? g.edges[k0 + j] : make_int2(-1, 0);
-#pragma unroll
-                for (int j = 0; j < kSliceU; j++) {
-                    if (ew[j].x < 0) break;
-                    col[(ew[j].x >> g.slice_shift) * kSliceTile]++;
-                    all_neg &= ew[j].y < 0;
-                    any_neg |= ew[j].y < 0;
-                }
-            }
-            // lift(0)(v) > 0: owner 0 with every edge negative, owner 1 with a negative edge
-            g.stamp[v] = ((v < g.n0) ? all_neg : any_neg) ? 1 : 0;
-        }
+        const bool big = valid && ve - vs > g.big_deg;
         const unsigned long long word = __ballot(big);
         if (lane == 0 && valid) bigmask[v >> 6] = word;
+        s_beg[threadIdx.x] = vs;
+        s_end[threadIdx.x] = big ? vs - 1 : ve;   // big vertices (end before start): the hub kernels' edges
+        for (int q = 0; q < P; q++) s_cnt[q * kSliceQ + threadIdx.x] = 0;
+        __syncthreads();
+        // count: every vertex's first G edges (kept), then the rest
+        int2 ew[G];
+        unsigned neg = 0, nonneg = 0;             // bit i: vertex i * kGroups + grp
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            const int vl = i * kGroups + grp;
+            const int k = s_beg[vl] + sub;
+            ew[i] = k < s_end[vl] ? g.edges[k] : make_int2(-1, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            if (ew[i].x < 0) continue;
+            atomicAdd(&s_cnt[(ew[i].x >> g.slice_shift) * kSliceQ + i * kGroups + grp], 1);
+            (ew[i].y < 0 ? neg : nonneg) |= 1u << i;
+        }
+        for (int i = 0; i < G; i++) {
+            const int vl = i * kGroups + grp;
+            const int e = s_end[vl];
+            for (int k = s_beg[vl] + G + sub; k < e; k += G) {
+                const int2 x = g.edges[k];
+                atomicAdd(&s_cnt[(x.x >> g.slice_shift) * kSliceQ + vl], 1);
+                (x.y < 0 ? neg : nonneg) |= 1u << i;
+            }
+        }
+#pragma unroll
+        for (int d = 1; d < G; d <<= 1) {
+            neg |= __shfl_xor(neg, d, 64);
+            nonneg |= __shfl_xor(nonneg, d, 64);
+        }
+        {   // lane sub of each group decides vertex sub * kGroups + grp
+            const int vl = sub * kGroups + grp;
+            const int64_t u = first + vl;
+            // lift(0)(u) > 0: owner 0 with every edge negative, owner 1 with a negative edge
+            if (u < g.n && s_end[vl] >= s_beg[vl])
+                g.stamp[u] = ((u < g.n0) ? !((nonneg >> sub) & 1) : ((neg >> sub) & 1)) ? 1 : 0;
+        }
+        __syncthreads();
         uint8_t *len_out = slen + (size_t)T * P * kSliceTile;
         for (int q = 0; q < P; q++) {
-            const int c = col[q * kSliceTile];
+            const int c = s_cnt[q * kSliceQ + threadIdx.x];
             len_out[q * kSliceTile + threadIdx.x] = (uint8_t)c;
-            int t = c;
-            for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
-            if (lane == 0) s_wt[q * kSliceWaves + wave] = t;
+            const int inc = egs_wave_incl_scan(c);
+            s_cnt[q * kSliceQ + threadIdx.x] = inc - c;         // the wave's exclusive prefix
+            if (lane == 63) s_wt[q * kSliceWaves + wave] = inc;
         }
         __syncthreads();
         // (slice, wave) sums scanned in that order: at most kMaxPhases * kSliceWaves = 64 items
@@ -675,22 +685,23 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
             }
         }
         __syncthreads();
-        for (int q = 0; q < P; q++) col[q * kSliceTile] = s_wt[q * kSliceWaves + wave] + wave_excl_scan(col[q * kSliceTile]);
-        if (mine) {
-            for (int k0 = s; k0 < e; k0 += kSliceU) {
-                int2 ew[kSliceU];
+        for (int q = 0; q < P; q++) s_cnt[q * kSliceQ + threadIdx.x] += s_wt[q * kSliceWaves + wave];
+        __syncthreads();
+        // place: each edge at its (slice, vertex) segment's next slot
+        auto put = [&](const int2 x, int vl) {
+            const int p = atomicAdd(&s_cnt[(x.x >> g.slice_shift) * kSliceQ + vl], 1);
+            if (g.sshift) ((uint32_t *)sedges)[p] = ((uint32_t)x.y << g.sshift) | (uint32_t)x.x;
+            else ((int2 *)sedges)[p] = x;
+        };
 #pragma unroll
-                for (int j = 0; j < kSliceU; j++) ew[j] = k0 + j < e ? g.edges[k0 + j] : make_int2(-1, 0);
-#pragma unroll
-                for (int j = 0; j < kSliceU; j++) {
-                    if (ew[j].x < 0) break;
-                    const int p = col[(ew[j].x >> g.slice_shift) * kSliceTile]++;
-                    if (g.sshift) ((uint32_t *)sedges)[p] = ((uint32_t)ew[j].y << g.sshift) | (uint32_t)ew[j].x;
-                    else ((int2 *)sedges)[p] = ew[j];
-                }
-            }
+        for (int i = 0; i < G; i++)
+            if (ew[i].x >= 0) put(ew[i], i * kGroups + grp);
+        for (int i = 0; i < G; i++) {
+            const int vl = i * kGroups + grp;
+            const int e = s_end[vl];
+            for (int k = s_beg[vl] + G + sub; k < e; k += G) put(g.edges[k], vl);
         }
-        __syncthreads();    // s_wt is rewritten for the next tile
+        __syncthreads();    // s_wt, s_beg, s_end and s_cnt are rewritten for the next tile
     }
 }
 
@@ -880,9 +891,19 @@ hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int j, hipStrea
 
 hipError_t launch_slices(const DevGame &g, hipStream_t st) {
     if (g.phases <= 1) return hipSuccess;
-    hipLaunchKernelGGL(k_slice_build, dim3(grid_for(g.n, kSliceTile, 4096)), dim3(kSliceTile), 0, st, g,
-                       const_cast<uint8_t *>(g.slen), const_cast<int *>(g.sbase), const_cast<void *>(g.sedges),
-                       const_cast<unsigned long long *>(g.bigmask));
+    // lanes per vertex: the average degree rounded up to 4, 8 or 16
+    const int64_t avg = (g.m + g.n - 1) / g.n;
+    const dim3 grid(grid_for(g.n, kSliceTile, 4096));
+    uint8_t *slen = const_cast<uint8_t *>(g.slen);
+    int *sbase = const_cast<int *>(g.sbase);
+    void *sedges = const_cast<void *>(g.sedges);
+    unsigned long long *bigmask = const_cast<unsigned long long *>(g.bigmask);
+    if (avg <= 4)
+        hipLaunchKernelGGL(k_slice_build<4>, grid, dim3(kSliceTile), 0, st, g, slen, sbase, sedges, bigmask);
+    else if (avg <= 8)
+        hipLaunchKernelGGL(k_slice_build<8>, grid, dim3(kSliceTile), 0, st, g, slen, sbase, sedges, bigmask);
+    else
+        hipLaunchKernelGGL(k_slice_build<16>, grid, dim3(kSliceTile), 0, st, g, slen, sbase, sedges, bigmask);
     return hipGetLastError();
 }
 
