@@ -538,12 +538,26 @@ __global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int j) {
             if (v < g.n && g.val[v] == EGS_TOP) settled |= 1u << k;
         }
     uint64_t edges = 0;
+    // the next (slice, tile)'s length byte and wave base load while this one folds
+    int len_n = 0, base_n = 0;
+    if (K > 0) {
+        const size_t seg = (size_t)blockIdx.x * P;
+        len_n = g.slen[seg * kSliceTile + threadIdx.x];
+        base_n = g.sbase[seg * kSliceWaves + wave];
+    }
     for (int s = 0; s < P; s++) {
         for (int k = 0; k < K; k++) {
             const int64_t T = blockIdx.x + k * W;
-            const size_t seg = (size_t)T * P + s;
-            const int len = g.slen[seg * kSliceTile + threadIdx.x];
-            const int b = g.sbase[seg * kSliceWaves + wave] + wave_excl_scan(len);
+            const int len = len_n;
+            const int b = base_n + wave_excl_scan(len);
+            {
+                const int kn = k + 1 < K ? k + 1 : 0, sn = k + 1 < K ? s : s + 1;
+                if (sn < P) {
+                    const size_t seg = (size_t)(blockIdx.x + kn * W) * P + sn;
+                    len_n = g.slen[seg * kSliceTile + threadIdx.x];
+                    base_n = g.sbase[seg * kSliceWaves + wave];
+                }
+            }
             const int e = ((settled >> k) & 1u) ? b : b + len;   // the prefix counts every lane's edges
             const bool owner1 = T * kSliceTile + threadIdx.x >= g.n0;
             int key = EGS_TOP;
