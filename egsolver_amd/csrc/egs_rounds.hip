@@ -328,17 +328,30 @@ template <int G>
 __device__ __forceinline__ void body_dense(const DevGame &g, const Stage &st, int r, LiftAcc &acc) {
     constexpr int ITEMS = 256 / G;
     const int gl = threadIdx.x % G;
+    const int64_t stride = (int64_t)g.round_grid * ITEMS;
+    // the next vertex's row bounds and value load while this one folds (its value cannot
+    // change meanwhile: only its own group writes it in a dense round)
+    int s_n = 0, e_n = 0, o_n = 0;
+    auto fetch = [&](int64_t t) {
+        const int64_t u = t + threadIdx.x / G;
+        if (u < g.n) {
+            s_n = g.ptr[u];
+            e_n = g.ptr[u + 1];
+            o_n = g.val[u];
+        }
+    };
     int it = 0;
-    for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < g.n; t0 += (int64_t)g.round_grid * ITEMS, it++) {
+    if ((int64_t)blockIdx.x * ITEMS < g.n) fetch((int64_t)blockIdx.x * ITEMS);
+    for (int64_t t0 = (int64_t)blockIdx.x * ITEMS; t0 < g.n; t0 += stride, it++) {
         const int v = (int)(t0 + threadIdx.x / G);
         bool valid = v < g.n;
-        int s = 0, e = 0, old = 0;
+        int s = s_n, e = e_n, old = o_n;
+        if (t0 + stride < g.n) fetch(t0 + stride);
         if (valid) {
-            s = g.ptr[v];
-            e = g.ptr[v + 1];
-            if (e - s > g.big_deg) { valid = false; s = e = 0; }   // static wave/block lists
-            else old = g.val[v];
+            if (e - s > g.big_deg) { valid = false; s = e = 0; old = 0; }   // static wave/block lists
             if (old == EGS_TOP) e = s;          // TOP is final: nothing to gather
+        } else {
+            s = e = old = 0;
         }
         const int t = lift_value<G>(g, s, e, gl, v >= g.n0);
         const bool lifted = valid && old < t;
