@@ -429,22 +429,49 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
     const int gbase = (threadIdx.x & 63) & ~(G - 1);
     const long long scale = (long long)g.n + 1;
     int local = 0, removed = 0;
-    for (int64_t base = (int64_t)blockIdx.x * kPer; base < g.n; base += (int64_t)gridDim.x * kPer) {
+    const int64_t stride = (int64_t)gridDim.x * kPer;
+    // the next vertex's potential and row bounds load while this one is evaluated (only its
+    // own group writes its potential in this pass)
+    long long h_n = kHOut;
+    int kb_n = 0, ke_n = 0;
+    auto fetch = [&](int64_t b) {
+        const int64_t u = b + threadIdx.x / G;
+        if (u < g.n) {
+            h_n = h[u];
+            kb_n = g.ptr[u];
+            ke_n = g.ptr[u + 1];
+        }
+    };
+    if ((int64_t)blockIdx.x * kPer < g.n) fetch((int64_t)blockIdx.x * kPer);
+    for (int64_t base = (int64_t)blockIdx.x * kPer; base < g.n; base += stride) {
         const int v = (int)base + (int)threadIdx.x / G;
-        const long long hv = v < g.n ? h[v] : kHOut;
+        const long long hv = v < g.n ? h_n : kHOut;
         const bool member = hv >= 0;
         const bool owner1 = v >= g.n0;
         int kb = 0, ke = 0;
+        const int kb_c = kb_n, ke_c = ke_n;
+        if (base + stride < g.n) fetch(base + stride);
         bool need = false;
+        // the lane's first kBfU edges, read once for the test and the evaluation
+        int2 ew0[kBfU];
+#pragma unroll
+        for (int j = 0; j < kBfU; j++) ew0[j] = make_int2(0, 0);
         if (member) {
-            kb = g.ptr[v];
-            ke = g.ptr[v + 1];
+            kb = kb_c;
+            ke = ke_c;
             if (ke - kb > g.big_deg) ke = kb;         // k_bf_big evaluates it: no need here
             // kBfU edges per lane at a time (an edge past ke re-tests edge k0, a valid index)
-            for (int k0 = kb + gl; k0 < ke && !need; k0 += G * kBfU) {
+            const int k0 = kb + gl;
+            if (k0 < ke) {
+#pragma unroll
+                for (int j = 0; j < kBfU; j++) ew0[j] = g.edges[k0 + j * G < ke ? k0 + j * G : k0];
+#pragma unroll
+                for (int j = 0; j < kBfU; j++) need |= bm_test(prev, g.n, ew0[j].x);
+            }
+            for (int k1 = k0 + G * kBfU; k1 < ke && !need; k1 += G * kBfU) {
                 int u[kBfU];
 #pragma unroll
-                for (int j = 0; j < kBfU; j++) u[j] = g.edges[k0 + j * G < ke ? k0 + j * G : k0].x;
+                for (int j = 0; j < kBfU; j++) u[j] = g.edges[k1 + j * G < ke ? k1 + j * G : k1].x;
 #pragma unroll
                 for (int j = 0; j < kBfU; j++) need |= bm_test(prev, g.n, u[j]);
             }
@@ -452,8 +479,21 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
         need = ((__ballot(need) >> gbase) & kGroupMask) != 0;
         long long best = owner1 ? LLONG_MAX : LLONG_MIN;
         bool out = false;                    // an edge leaves D and does not reach TOP
+        if (member && need && kb + gl < ke) {
+            const int k0 = kb + gl;
+            long long hu[kBfU];
+#pragma unroll
+            for (int j = 0; j < kBfU; j++) hu[j] = k0 + j * G < ke ? h[ew0[j].x] : kHTop;
+#pragma unroll
+            for (int j = 0; j < kBfU; j++) {
+                if (k0 + j * G >= ke) break;
+                if (hu[j] == kHOut) { out = true; continue; }
+                const long long c = hu[j] == kHTop ? LLONG_MIN : scale * ew0[j].y + 1 + hu[j];
+                best = owner1 ? min(best, c) : max(best, c);
+            }
+        }
         if (member && need) {
-            for (int k0 = kb + gl; k0 < ke; k0 += G * kBfU) {   // loads, then gathers, in flight together
+            for (int k0 = kb + gl + G * kBfU; k0 < ke; k0 += G * kBfU) {   // loads, then gathers, in flight together
                 int2 ew[kBfU];
                 long long hu[kBfU];
 #pragma unroll
