@@ -105,14 +105,29 @@ __device__ __forceinline__ int game_sweep_part(const GameDesc &b, const GameVals
     const int gid = threadIdx.x / G;
     const int ngroups = blockDim.x / G;
     int base = cursor;
+    // the next chunk's offsets load while this chunk runs (they never change)
+    int s_n = 0, e_n = 0, c0_n = 0, c1_n = 0;
+    auto fetch = [&](int at) {
+        const int u = at + gid;
+        if (u < b.n) {
+            s_n = b.ptr[u];
+            e_n = b.ptr[u + 1];
+        }
+        c0_n = b.ptr[at];
+        c1_n = b.ptr[min(at + ngroups, b.n)];
+    };
+    if (base < b.n) fetch(base);
     while (base < b.n && spent < budget) {
-        spent += b.ptr[min(base + ngroups, b.n)] - b.ptr[base];
+        spent += c1_n - c0_n;
         const int v = base + gid;
         const bool valid = v < b.n;
         int s = 0, e = 0, old = 0;
         if (valid) {
-            s = b.ptr[v];
-            e = b.ptr[v + 1];
+            s = s_n;
+            e = e_n;
+        }
+        if (base + ngroups < b.n) fetch(base + ngroups);
+        if (valid) {
             old = gv<SPLIT>(V, v);
             if (old == EGS_TOP) e = s;          // TOP is final: nothing to gather
         }
