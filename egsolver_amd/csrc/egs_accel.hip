@@ -414,22 +414,22 @@ __global__ __launch_bounds__(256) void k_closure_big(DevGame g, int B, int last,
 // `clear` (the bitmap of pass p+1, last read in pass p-1) is zeroed.  The pass returns at
 // once when pass p-1 changed nothing or the potentials have passed the divergence bound.
 template <int G>
-__global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restrict__ h, long long bound,
-                                                 const unsigned *__restrict__ prev, unsigned *__restrict__ cur,
-                                                 unsigned *__restrict__ clear, int p, int *__restrict__ cnt,
-                                                 int *__restrict__ flags) {
+__device__ __forceinline__ void bf_pass_part(const DevGame &g, long long *__restrict__ h, long long bound,
+                                             const unsigned *__restrict__ prev, unsigned *__restrict__ cur,
+                                             unsigned *__restrict__ clear, int p, int *__restrict__ cnt,
+                                             int *__restrict__ flags, int bid, int nblk) {
     if (cnt[(p - 1) % kCntSlots] == 0 || flags[1]) return;
     const int64_t words = bm_total(g.n);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t i = (int64_t)bid * blockDim.x + threadIdx.x; i < words; i += (int64_t)nblk * blockDim.x)
         clear[i] = 0u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[(p + 1) % kCntSlots] = 0;
+    if (bid == 0 && threadIdx.x == 0) cnt[(p + 1) % kCntSlots] = 0;
     constexpr int kPer = 256 / G;
     constexpr unsigned long long kGroupMask = (G == 64) ? ~0ull : ((1ull << G) - 1);
     const int gl = threadIdx.x % G;
     const int gbase = (threadIdx.x & 63) & ~(G - 1);
     const long long scale = (long long)g.n + 1;
     int local = 0, removed = 0;
-    const int64_t stride = (int64_t)gridDim.x * kPer;
+    const int64_t stride = (int64_t)nblk * kPer;
     // the next vertex's potential and row bounds load while this one is evaluated (only its
     // own group writes its potential in this pass)
     long long h_n = kHOut;
@@ -442,8 +442,8 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
             ke_n = g.ptr[u + 1];
         }
     };
-    if ((int64_t)blockIdx.x * kPer < g.n) fetch((int64_t)blockIdx.x * kPer);
-    for (int64_t base = (int64_t)blockIdx.x * kPer; base < g.n; base += stride) {
+    if ((int64_t)bid * kPer < g.n) fetch((int64_t)bid * kPer);
+    for (int64_t base = (int64_t)bid * kPer; base < g.n; base += stride) {
         const int v = (int)base + (int)threadIdx.x / G;
         const long long hv = v < g.n ? h_n : kHOut;
         const bool member = hv >= 0;
@@ -534,17 +534,25 @@ __global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restric
     block_add(removed, &flags[0]);
 }
 
+template <int G>
+__global__ __launch_bounds__(256) void k_bf_pass(DevGame g, long long *__restrict__ h, long long bound,
+                                                 const unsigned *__restrict__ prev, unsigned *__restrict__ cur,
+                                                 unsigned *__restrict__ clear, int p, int *__restrict__ cnt,
+                                                 int *__restrict__ flags) {
+    bf_pass_part<G>(g, h, bound, prev, cur, clear, p, cnt, flags, blockIdx.x, gridDim.x);
+}
+
 // Pass p for the vertices of the big list (big_deg < deg <= huge_deg; launched after
 // k_bf_pass for the same p), one workgroup each, with the same rule and updates.
-__global__ __launch_bounds__(256) void k_bf_big(DevGame g, long long *__restrict__ h, long long bound,
-                                                const unsigned *__restrict__ prev, unsigned *__restrict__ cur,
-                                                int p, int *__restrict__ cnt, int *__restrict__ flags) {
+__device__ __forceinline__ void bf_big_part(const DevGame &g, long long *__restrict__ h, long long bound,
+                                            const unsigned *__restrict__ prev, unsigned *__restrict__ cur, int p,
+                                            int *__restrict__ cnt, int *__restrict__ flags, int bid, int nblk) {
     __shared__ long long sm[4];
     if (cnt[(p - 1) % kCntSlots] == 0 || flags[1]) return;
     const long long scale = (long long)g.n + 1;
     const int nb = g.ring[9], items = nb;               // the huge list: k_bf_huge
     int local = 0;
-    for (int64_t i = blockIdx.x; i < items; i += gridDim.x) {
+    for (int64_t i = bid; i < items; i += nblk) {
         const int v = egs_big_item(g.sbig, g.shuge, nb, items - 1 - i);   // huge vertices first
         const long long hv = h[v];
         if (hv < 0) continue;                           // not in D: the whole workgroup
@@ -599,6 +607,12 @@ __global__ __launch_bounds__(256) void k_bf_big(DevGame g, long long *__restrict
     if (threadIdx.x == 0 && local) atomicAdd(&cnt[p % kCntSlots], local);
 }
 
+__global__ __launch_bounds__(256) void k_bf_big(DevGame g, long long *__restrict__ h, long long bound,
+                                                const unsigned *__restrict__ prev, unsigned *__restrict__ cur,
+                                                int p, int *__restrict__ cnt, int *__restrict__ flags) {
+    bf_big_part(g, h, bound, prev, cur, p, cnt, flags, blockIdx.x, gridDim.x);
+}
+
 // Pass p for the vertices above huge_deg, each shared by up to `per` workgroups: a
 // workgroup takes every per-th chunk of kHugeChunkBf edges and folds into the vertex's
 // accumulators whether a successor changed, whether an edge leaves D, and the min (owner 1)
@@ -607,9 +621,9 @@ __global__ __launch_bounds__(256) void k_bf_big(DevGame g, long long *__restrict
 // contributors read h[v] before any of them counts in, so they agree on membership.
 constexpr int kHugeChunkBf = 2048;
 
-__global__ __launch_bounds__(256) void k_bf_huge(DevGame g, long long *__restrict__ h, long long bound,
-                                                 const unsigned *__restrict__ prev, unsigned *__restrict__ cur,
-                                                 int p, int *__restrict__ cnt, int *__restrict__ flags) {
+__device__ __forceinline__ void bf_huge_part(const DevGame &g, long long *__restrict__ h, long long bound,
+                                             const unsigned *__restrict__ prev, unsigned *__restrict__ cur, int p,
+                                             int *__restrict__ cnt, int *__restrict__ flags, int bid, int nblk) {
     __shared__ long long sm[4];
     __shared__ long long s_best;
     __shared__ int s_last, s_flag;
@@ -617,10 +631,10 @@ __global__ __launch_bounds__(256) void k_bf_huge(DevGame g, long long *__restric
     const int nh = g.ring[10];
     if (nh == 0) return;
     const long long scale = (long long)g.n + 1;
-    const int per = max(1, (int)gridDim.x / nh);
+    const int per = max(1, nblk / nh);
     const int64_t units = (int64_t)nh * per;
     int local = 0, removed = 0;
-    for (int64_t u = blockIdx.x; u < units; u += gridDim.x) {
+    for (int64_t u = bid; u < units; u += nblk) {
         const int i = (int)(u % nh), part = (int)(u / nh);
         const int v = g.shuge[nh - 1 - i];                 // largest first
         const long long hv = h[v];
@@ -696,6 +710,28 @@ __global__ __launch_bounds__(256) void k_bf_huge(DevGame g, long long *__restric
     if (threadIdx.x == 0 && removed) atomicAdd(&flags[0], removed);
 }
 
+__global__ __launch_bounds__(256) void k_bf_huge(DevGame g, long long *__restrict__ h, long long bound,
+                                                 const unsigned *__restrict__ prev, unsigned *__restrict__ cur,
+                                                 int p, int *__restrict__ cnt, int *__restrict__ flags) {
+    bf_huge_part(g, h, bound, prev, cur, p, cnt, flags, blockIdx.x, gridDim.x);
+}
+
+// One pass over every member in one launch: workgroups [0, grid) run the pass over the
+// vertices up to big_deg, the next big_grid the big list and the last huge_grid the huge
+// list.  The three touch disjoint potentials, add to the same pass counters and write
+// their changes to the same bitmap, so they need no order among themselves: each part
+// strides over its own workgroups only.
+template <int G>
+__global__ __launch_bounds__(256) void k_bf_all(DevGame g, long long *__restrict__ h, long long bound,
+                                                const unsigned *__restrict__ prev, unsigned *__restrict__ cur,
+                                                unsigned *__restrict__ clear, int p, int *__restrict__ cnt,
+                                                int *__restrict__ flags, int grid, int big_grid, int huge_grid) {
+    const int b = blockIdx.x;
+    if (b < grid) bf_pass_part<G>(g, h, bound, prev, cur, clear, p, cnt, flags, b, grid);
+    else if (b < grid + big_grid) bf_big_part(g, h, bound, prev, cur, p, cnt, flags, b - grid, big_grid);
+    else bf_huge_part(g, h, bound, prev, cur, p, cnt, flags, b - grid - big_grid, huge_grid);
+}
+
 // Starting state of the vertices above huge_deg, in one pass: the shared lifts' partial keys
 // and chunk counts (TOP and 0), and the accumulators of k_bf_huge (the identity of each
 // owner's fold) with their flags.
@@ -720,6 +756,19 @@ __global__ __launch_bounds__(256) void k_mark_top(DevGame g, const long long *__
         if (h[v] >= 0) { g.val[v] = EGS_TOP; g.ring[15] = 1; local++; }
     }
     block_add(local, count);
+}
+
+static void launch_bf_all(int G, int grid, int big_grid, int huge_grid, hipStream_t st, const DevGame &g,
+                          long long *h, long long bound, const unsigned *prev, unsigned *cur, unsigned *clear, int p,
+                          int *cnt, int *flags) {
+    const dim3 all(grid + big_grid + huge_grid), t(256);
+    switch (G) {
+    case 1: hipLaunchKernelGGL(k_bf_all<1>, all, t, 0, st, g, h, bound, prev, cur, clear, p, cnt, flags, grid, big_grid, huge_grid); break;
+    case 2: hipLaunchKernelGGL(k_bf_all<2>, all, t, 0, st, g, h, bound, prev, cur, clear, p, cnt, flags, grid, big_grid, huge_grid); break;
+    case 4: hipLaunchKernelGGL(k_bf_all<4>, all, t, 0, st, g, h, bound, prev, cur, clear, p, cnt, flags, grid, big_grid, huge_grid); break;
+    case 8: hipLaunchKernelGGL(k_bf_all<8>, all, t, 0, st, g, h, bound, prev, cur, clear, p, cnt, flags, grid, big_grid, huge_grid); break;
+    default: hipLaunchKernelGGL(k_bf_all<16>, all, t, 0, st, g, h, bound, prev, cur, clear, p, cnt, flags, grid, big_grid, huge_grid); break;
+    }
 }
 
 static void launch_bf_pass(int G, int grid, hipStream_t st, const DevGame &g, long long *h, long long bound,
@@ -859,14 +908,13 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
             if (stop && *stop) return 0;    // abandon the candidate: nothing was marked
             const int upto = std::min(cap, queued == 1 ? first_chunk : queued + chunk);
             for (int p = queued; p < upto; p++) {
-                launch_bf_pass(G, bf_grid, st, g, cb.h, bound, bm[(p - 1) % 3], bm[p % 3], bm[(p + 1) % 3], p, cb.bfcnt,
-                               flags);
-                if (has_big) {
-                    hipLaunchKernelGGL(k_bf_big, dim3(kBigGrid), dim3(256), 0, st, g, cb.h, bound, bm[(p - 1) % 3],
-                                       bm[p % 3], p, cb.bfcnt, flags);
-                    hipLaunchKernelGGL(k_bf_huge, dim3(2 * kBigGrid), dim3(256), 0, st, g, cb.h, bound,
-                                       bm[(p - 1) % 3], bm[p % 3], p, cb.bfcnt, flags);
-                }
+                // with vertices above big_deg, their parts of the pass run in the same launch
+                if (has_big)
+                    launch_bf_all(G, bf_grid, kBigGrid, 2 * kBigGrid, st, g, cb.h, bound, bm[(p - 1) % 3], bm[p % 3],
+                                  bm[(p + 1) % 3], p, cb.bfcnt, flags);
+                else
+                    launch_bf_pass(G, bf_grid, st, g, cb.h, bound, bm[(p - 1) % 3], bm[p % 3], bm[(p + 1) % 3], p,
+                                   cb.bfcnt, flags);
             }
             if (hipMemcpyAsync(fl, flags, sizeof(int) * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
             if (hipMemcpyAsync(cnt, cb.bfcnt, sizeof(int) * kCntSlots, hipMemcpyDeviceToHost, st) != hipSuccess)
