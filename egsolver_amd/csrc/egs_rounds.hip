@@ -302,15 +302,20 @@ __global__ __launch_bounds__(256) void k_init_big(DevGame g) {
     for (int64_t i = blockIdx.x; i < items; i += gridDim.x) {
         const int v = egs_big_item(g.sbig, g.shuge, nb, i);
         const int s = g.ptr[v], e = g.ptr[v + 1];
-        bool all_neg = true, any_neg = false;
-        for (int k = s + (int)threadIdx.x; k < e; k += blockDim.x) {
-            const bool neg = g.edges[k].y < 0;
-            all_neg &= neg;
-            any_neg |= neg;
+        // one edge decides: a non-negative weight for owner 0 (not all negative), a negative
+        // one for owner 1; the workgroup stops at the first block of edges holding one
+        const bool owner1 = v >= g.n0;
+        bool found = false;
+        for (int k0 = s; k0 < e; k0 += blockDim.x) {
+            const int k = k0 + (int)threadIdx.x;
+            bool hit = false;
+            if (k < e) hit = owner1 ? g.edges[k].y < 0 : g.edges[k].y >= 0;
+            if (__syncthreads_or(hit)) {
+                found = true;
+                break;
+            }
         }
-        all_neg = __syncthreads_and(all_neg);
-        any_neg = __syncthreads_or(any_neg);
-        const bool act = (v < g.n0) ? all_neg : any_neg;
+        const bool act = owner1 ? found : !found;
         if (threadIdx.x < 64) {
             if (threadIdx.x == 0 && act) g.stamp[v] = 1;
             push_wave(g, g.list[1], g.lcnt + 1 * kListCounters, (int)(blockIdx.x % kSub), act && threadIdx.x == 0, v);
@@ -872,7 +877,7 @@ __global__ __launch_bounds__(256) void k_lift_block(DevGame g, int j) {
 // The counters, modes and ring slots start cleared (k_prepare_reset).
 hipError_t launch_init(const DevGame &g, hipStream_t st) {
     hipLaunchKernelGGL(k_init, dim3(grid_for(g.n, 256, 2048)), dim3(256), 0, st, g);
-    hipLaunchKernelGGL(k_init_big, dim3(256), dim3(256), 0, st, g);
+    hipLaunchKernelGGL(k_init_big, dim3(1024), dim3(256), 0, st, g);
     return hipGetLastError();
 }
 

@@ -140,17 +140,52 @@ __global__ __launch_bounds__(256) void k_scatter_preds(const int *__restrict__ p
 
 // Static lists of high-degree vertices, used by dense sweeps: big_deg < deg <= huge_deg
 // go to the wavefront kernel, deg > huge_deg to the workgroup kernel.  One-time
-// compaction with wave-aggregated appends.
-__global__ __launch_bounds__(256) void k_classify(const int *__restrict__ ptr, int n, int big_deg,
-                                                  int huge_deg, int *__restrict__ sbig,
-                                                  int *__restrict__ shuge, int *__restrict__ counts) {
-    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x; v0 < n; v0 += (int64_t)gridDim.x * blockDim.x) {
-        const int v = (int)(v0 + threadIdx.x);
-        int d = 0;
-        if (v < n) d = ptr[v + 1] - ptr[v];
-        egs_append(sbig, &counts[0], v < n && d > big_deg && d <= huge_deg, v);
-        egs_append(shuge, &counts[1], v < n && d > huge_deg, v);
+// compaction: a workgroup gathers its vertices of each list in LDS (wave-aggregated) and
+// reserves their places with one global atomic per list, since thousands of appends to
+// the same two counters wait on each other (60 us on the skewed 2^20 game).  What does not
+// fit the LDS buffers goes out directly.
+constexpr int kClassifyLds = 1024;   // list entries a workgroup holds per list
+
+__global__ __launch_bounds__(1024) void k_classify(const int *__restrict__ ptr, int n, int big_deg, int huge_deg,
+                                                   int *__restrict__ sbig, int *__restrict__ shuge,
+                                                   int *__restrict__ counts) {
+    __shared__ int lbig[kClassifyLds], lhuge[kClassifyLds];
+    __shared__ int cbig, chuge, obig, ohuge;
+    if (threadIdx.x == 0) { cbig = 0; chuge = 0; }
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int d_n = 0;                              // the next step's degree loads while this one runs
+    if ((int64_t)blockIdx.x * blockDim.x + threadIdx.x < n) {
+        const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        d_n = ptr[u + 1] - ptr[u];
     }
+    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x; v0 < n; v0 += stride) {
+        const int v = (int)(v0 + threadIdx.x);
+        const int d = d_n;
+        if (v0 + stride + threadIdx.x < n) d_n = ptr[v0 + stride + threadIdx.x + 1] - ptr[v0 + stride + threadIdx.x];
+        const bool big = v < n && d > big_deg && d <= huge_deg, huge = v < n && d > huge_deg;
+        // a place in the workgroup's buffer, or straight to the global list when it is full
+        const uint64_t mb = __ballot(big), mh = __ballot(huge);
+        const int lane = threadIdx.x & 63;
+        int bb = 0, bh = 0;
+        if (lane == 0 && mb) bb = atomicAdd(&cbig, __popcll(mb));
+        if (lane == 0 && mh) bh = atomicAdd(&chuge, __popcll(mh));
+        bb = __shfl(bb, 0, 64) + __popcll(mb & egs_lanemask_lt());
+        bh = __shfl(bh, 0, 64) + __popcll(mh & egs_lanemask_lt());
+        if (big && bb < kClassifyLds) lbig[bb] = v;
+        if (huge && bh < kClassifyLds) lhuge[bh] = v;
+        egs_append(sbig, &counts[0], big && bb >= kClassifyLds, v);
+        egs_append(shuge, &counts[1], huge && bh >= kClassifyLds, v);
+    }
+    __syncthreads();
+    const int nbig = min(cbig, kClassifyLds), nhuge = min(chuge, kClassifyLds);
+    if (threadIdx.x == 0) {
+        obig = nbig ? atomicAdd(&counts[0], nbig) : 0;
+        ohuge = nhuge ? atomicAdd(&counts[1], nhuge) : 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbig; i += blockDim.x) sbig[obig + i] = lbig[i];
+    for (int i = threadIdx.x; i < nhuge; i += blockDim.x) shuge[ohuge + i] = lhuge[i];
 }
 
 // Fixpoint certificate: count vertices with lift(val)(v) != val(v).  A lane takes its own
@@ -322,7 +357,7 @@ hipError_t launch_prepare_reset(const DevGame &g, hipStream_t st) {
 
 // counts[0..1] (ring[9], ring[10]) start at zero (k_prepare_reset).
 hipError_t launch_classify(const DevGame &g, int *counts, hipStream_t st) {
-    hipLaunchKernelGGL(k_classify, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, st, g.ptr, g.n, g.big_deg,
+    hipLaunchKernelGGL(k_classify, dim3(grid_for(g.n, 1024, 512)), dim3(1024), 0, st, g.ptr, g.n, g.big_deg,
                        g.huge_deg, const_cast<int *>(g.sbig), const_cast<int *>(g.shuge), counts);
     return hipGetLastError();
 }
