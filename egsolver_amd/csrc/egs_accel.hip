@@ -344,62 +344,117 @@ __global__ __launch_bounds__(256) void k_closure_fused(DevGame g, int B, int las
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[1] = 0;
 }
 
-// k_closure for the vertices above big_deg (launched after it), one workgroup each: the same
-// closure test and first pass; changes set their bit with an atomic OR, since k_closure has
-// stored every word already.
+// k_closure for the vertices above big_deg (launched after it): the same closure test and
+// first pass; changes set their bit with an atomic OR, since k_closure has stored every
+// word already.  Workgroups [0, big_grid) take the big list, one vertex each.  The vertices
+// above huge_deg are shared by up to `per` of the last huge_grid workgroups, each folding
+// every per-th chunk of kHugeChunkCl edges into the vertex's accumulators (hflag bit 0: an
+// edge reaches D or TOP, bit 1: an edge leaves D and does not reach TOP; hbest: the min or
+// max candidate); the workgroup that completes hcnt[v] applies the update and resets them.
+// One workgroup walking a 70,000-edge hub alone took 135-265 us.
+constexpr int kHugeChunkCl = 2048;
+
+__device__ __forceinline__ void closure_fold(const DevGame &g, int k0, int e, int B, int last, bool owner1,
+                                             long long scale, long long &best, bool &any, bool &all) {
+    for (int k = k0 + (int)threadIdx.x; k < e; k += kBigU * 256) {
+        int2 ew[kBigU];
+        int xu[kBigU];
+#pragma unroll
+        for (int j = 0; j < kBigU; j++) ew[j] = k + 256 * j < e ? g.edges[k + 256 * j] : make_int2(0, 0);
+#pragma unroll
+        for (int j = 0; j < kBigU; j++) xu[j] = g.val[ew[j].x];
+#pragma unroll
+        for (int j = 0; j < kBigU; j++) {
+            if (k + 256 * j >= e) break;
+            if (xu[j] == EGS_TOP) {
+                any = true;
+                if (owner1) best = LLONG_MIN;
+            } else if (in_d_x(g, ew[j].x, xu[j], B, last)) {
+                any = true;
+                const long long c = scale * ew[j].y + 1 + scale * ((long long)g.mg - xu[j]);
+                best = owner1 ? min(best, c) : max(best, c);
+            } else {
+                all = false;
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_closure_big(DevGame g, int B, int last, long long bound,
                                                      long long *__restrict__ h, unsigned *__restrict__ bits,
-                                                     int *__restrict__ cnt, int *__restrict__ flags) {
+                                                     int *__restrict__ cnt, int *__restrict__ flags, int big_grid,
+                                                     int huge_grid) {
     __shared__ long long sm[4];
+    __shared__ long long s_best;
+    __shared__ int s_last, s_flag;
     const long long scale = (long long)g.n + 1;
-    const int nb = g.ring[9], items = nb + g.ring[10];
-    for (int64_t i = blockIdx.x; i < items; i += gridDim.x) {
-        const int v = egs_big_item(g.sbig, g.shuge, nb, items - 1 - i);   // huge vertices first
+    const bool big_part = (int)blockIdx.x < big_grid;
+    const int nb = g.ring[9], nh = g.ring[10];
+    const int nblk = big_part ? big_grid : huge_grid;
+    const int bid = big_part ? (int)blockIdx.x : (int)blockIdx.x - big_grid;
+    const int per = big_part ? 1 : max(1, huge_grid / max(nh, 1));
+    const int64_t units = big_part ? (int64_t)nb : (int64_t)nh * per;
+    for (int64_t u = bid; u < units; u += nblk) {
+        const int i = big_part ? (int)u : (int)(u % nh), part = big_part ? 0 : (int)(u / nh);
+        const int v = big_part ? g.sbig[i] : g.shuge[nh - 1 - i];          // the largest hubs first
         const int x = g.val[v];
         if (!in_d_x(g, v, x, B, last)) continue;        // the whole workgroup: v is shared
+        const int kb = g.ptr[v], ke = g.ptr[v + 1];
+        const int chunks = big_part ? 1 : (ke - kb + kHugeChunkCl - 1) / kHugeChunkCl;
+        const int contrib = min(per, max(chunks, 1));
+        if (part >= contrib) continue;
         const bool owner1 = v >= g.n0;
         long long best = owner1 ? LLONG_MAX : LLONG_MIN;
         bool any = false, all = true;
-        const int e = g.ptr[v + 1];
-        for (int k0 = g.ptr[v] + (int)threadIdx.x; k0 < e; k0 += kBigU * 256) {
-            int2 ew[kBigU];
-            int xu[kBigU];
-#pragma unroll
-            for (int j = 0; j < kBigU; j++) ew[j] = k0 + 256 * j < e ? g.edges[k0 + 256 * j] : make_int2(0, 0);
-#pragma unroll
-            for (int j = 0; j < kBigU; j++) xu[j] = g.val[ew[j].x];
-#pragma unroll
-            for (int j = 0; j < kBigU; j++) {
-                if (k0 + 256 * j >= e) break;
-                if (xu[j] == EGS_TOP) {
-                    any = true;
-                    if (owner1) best = LLONG_MIN;
-                } else if (in_d_x(g, ew[j].x, xu[j], B, last)) {
-                    any = true;
-                    const long long c = scale * ew[j].y + 1 + scale * ((long long)g.mg - xu[j]);
-                    best = owner1 ? min(best, c) : max(best, c);
-                } else {
-                    all = false;
-                }
+        if (big_part) {
+            closure_fold(g, kb, ke, B, last, owner1, scale, best, any, all);
+        } else {
+            for (int c = part; c < chunks; c += contrib) {
+                const int cs = kb + c * kHugeChunkCl;
+                closure_fold(g, cs, min(ke, cs + kHugeChunkCl), B, last, owner1, scale, best, any, all);
             }
         }
         any = __syncthreads_or(any);
         all = __syncthreads_and(all);
         best = egs_block_minmax_ll(best, owner1, sm);
         if (threadIdx.x == 0) {
+            bool done = true;
+            int f = (any ? 1 : 0) | (all ? 0 : 2);
+            if (contrib > 1) {
+                if (f) atomicOr(&g.hflag[v], f);
+                if (owner1) atomicMin(&g.hbest[v], best);
+                else atomicMax(&g.hbest[v], best);
+                __threadfence();
+                done = atomicAdd(&g.hcnt[v], 1) == contrib - 1;
+                if (done) {
+                    __threadfence();
+                    f = atomicExch(&g.hflag[v], 0);
+                    best = atomicExch((unsigned long long *)&g.hbest[v],
+                                      (unsigned long long)(owner1 ? LLONG_MAX : LLONG_MIN));
+                    g.hcnt[v] = 0;
+                }
+            }
+            s_last = done;
+            s_flag = f;
+            s_best = best;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && s_last) {
             const long long h0 = scale * ((long long)g.mg - x);
-            if (owner1 ? !any : !all) {
+            const long long b2 = s_best;
+            if (owner1 ? !(s_flag & 1) : (s_flag & 2)) {
                 h[v] = kHOut;
                 bm_set(bits, g.n, v);
                 atomicAdd(&cnt[0], 1);
                 atomicAdd(&flags[0], 1);
-            } else if (best > h0) {
-                h[v] = best;
+            } else if (b2 > h0) {
+                h[v] = b2;
                 bm_set(bits, g.n, v);
                 atomicAdd(&cnt[0], 1);
-                if (best > bound) atomicOr(&flags[1], 1);
+                if (b2 > bound) atomicOr(&flags[1], 1);
             }
         }
+        __syncthreads();
     }
 }
 
@@ -899,8 +954,8 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
                                flags);
         }
         if (has_big)
-            hipLaunchKernelGGL(k_closure_big, dim3(kBigGrid), dim3(256), 0, st, g, B, last, bound, cb.h, bm[0], cb.bfcnt,
-                               flags);
+            hipLaunchKernelGGL(k_closure_big, dim3(3 * kBigGrid), dim3(256), 0, st, g, B, last, bound, cb.h, bm[0],
+                               cb.bfcnt, flags, kBigGrid, 2 * kBigGrid);
         int queued = 1;                     // passes queued so far: pass 0 is the closure's
         int passes = 0;
         bool converged = false, diverged = false, rejected = false;
