@@ -870,6 +870,23 @@ __global__ __launch_bounds__(256) void k_lift_block(DevGame g, int j) {
     hub_block(g, r, mode, (int)blockIdx.x, (int)gridDim.x, sbuf, &scnt, wmin, &s_key, &s_last);
 }
 
+// Both hub lifts of a round in one launch, for games with both lists: workgroups
+// [0, wave_grid) run k_lift_wave's body, the next block_grid k_lift_block's.  They lift
+// disjoint vertices (as the hub part of a fused round kernel does), so one launch serves
+// both and a round whose hubs were lifted by the round kernel pays for one empty launch.
+__global__ __launch_bounds__(256) void k_lift_hubs(DevGame g, int j, int wave_grid, int block_grid) {
+    const int r = g.ring[14] + j;
+    __shared__ int sbuf[kLdsCap];
+    __shared__ int scnt;
+    __shared__ int wmin[4];
+    __shared__ int s_key, s_last;
+    const int mode = g.mode[r % 3];
+    if (mode == MODE_DONE) return;
+    if (mode == MODE_DENSE && g.hub_fused) return;     // the round kernel's hub part did it
+    if ((int)blockIdx.x < wave_grid) hub_wave(g, r, mode, (int)blockIdx.x, wave_grid, sbuf, &scnt);
+    else hub_block(g, r, mode, (int)blockIdx.x - wave_grid, block_grid, sbuf, &scnt, wmin, &s_key, &s_last);
+}
+
 // ---------------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------------
@@ -916,8 +933,13 @@ hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int j, hipStrea
         else
             hipLaunchKernelGGL(k_dense_fused<false>, dim3(g.phase_grid), dim3(256), lds, st, g, j);
     }
-    if (sh.use_wave) hipLaunchKernelGGL(k_lift_wave, dim3(sh.wave_grid), dim3(256), 0, st, g, j);
-    if (sh.use_block) hipLaunchKernelGGL(k_lift_block, dim3(sh.block_grid), dim3(256), 0, st, g, j);
+    if (sh.use_wave && sh.use_block)
+        hipLaunchKernelGGL(k_lift_hubs, dim3(sh.wave_grid + sh.block_grid), dim3(256), 0, st, g, j, sh.wave_grid,
+                           sh.block_grid);
+    else if (sh.use_wave)
+        hipLaunchKernelGGL(k_lift_wave, dim3(sh.wave_grid), dim3(256), 0, st, g, j);
+    else if (sh.use_block)
+        hipLaunchKernelGGL(k_lift_block, dim3(sh.block_grid), dim3(256), 0, st, g, j);
     return hipGetLastError();
 }
 
