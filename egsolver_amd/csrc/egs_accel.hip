@@ -37,18 +37,43 @@ constexpr int kBins = 64;
 // Window and floor histograms of the values against the candidates B_j = W * 2^j:
 // win[j]  counts f in (B_j - W, B_j]    (the value with ceil(f/W) == 2^j)
 // flo[k]  counts f with B_k < f <= B_{k+1}, so #{f > B_j, f < TOP} = sum_{k >= j} flo[k]
+// A lane's count added to its bin once per wave: the lanes whose bins match (ballots over the
+// bin's six bits) add through their lowest lane, so the few bins the values crowd into do not
+// serialise 64 LDS atomics per wave.  bin < 0: nothing to add.
+__device__ __forceinline__ void hist_add_wave(int *bins, int bin) {
+    unsigned long long eq = __builtin_amdgcn_read_exec();
+#pragma unroll
+    for (int b = 0; b < 7; b++) {
+        const bool bit = ((unsigned)bin >> b) & 1u;    // bit 6: the sign of "nothing"
+        const unsigned long long m = __ballot(bit);
+        eq &= bit ? m : ~m;
+    }
+    const int lane = threadIdx.x & 63;
+    if (bin >= 0 && (eq & ((1ull << lane) - 1)) == 0) atomicAdd(&bins[bin], __popcll(eq));
+}
+
+constexpr int kHistU = 8;   // values in flight per thread
+
 __global__ __launch_bounds__(256) void k_gap_hist(DevGame g, int wmax, int *__restrict__ out) {
     __shared__ int win[kBins], flo[kBins];
     for (int i = threadIdx.x; i < kBins; i += blockDim.x) { win[i] = 0; flo[i] = 0; }
     __syncthreads();
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
-        const int x = g.val[v];
-        if (x == EGS_TOP || x <= 0) continue;
-        const int64_t q = ((int64_t)x + wmax - 1) / wmax;          // ceil(x / W) >= 1
-        if ((q & (q - 1)) == 0) atomicAdd(&win[63 - __clzll((unsigned long long)q)], 1);
-        if (x > wmax) {
-            const int64_t r = ((int64_t)x - 1) / wmax;             // >= 1: largest k with W 2^k < x
-            atomicAdd(&flo[63 - __clzll((unsigned long long)r)], 1);
+    const int64_t threads = (int64_t)gridDim.x * blockDim.x;
+    const unsigned w = (unsigned)wmax;
+    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v0 < g.n; v0 += kHistU * threads) {
+        int x[kHistU];
+#pragma unroll
+        for (int j = 0; j < kHistU; j++) x[j] = v0 + j * threads < g.n ? g.val[v0 + j * threads] : 0;
+#pragma unroll
+        for (int j = 0; j < kHistU; j++) {
+            int bw = -1, bf = -1;
+            if (x[j] != EGS_TOP && x[j] > 0) {
+                const unsigned q = ((unsigned)x[j] + w - 1u) / w;      // ceil(x / W) >= 1 (x, W < 2^31)
+                if ((q & (q - 1)) == 0) bw = 31 - __clz(q);
+                if ((unsigned)x[j] > w) bf = 31 - __clz(((unsigned)x[j] - 1u) / w);   // largest k with W 2^k < x
+            }
+            hist_add_wave(win, bw);
+            hist_add_wave(flo, bf);
         }
     }
     __syncthreads();
@@ -808,8 +833,10 @@ hipError_t launch_hbest_reset(const DevGame &g, hipStream_t st) {
 __global__ __launch_bounds__(256) void k_mark_top(DevGame g, const long long *__restrict__ h, int *__restrict__ count) {
     int local = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
-        if (h[v] >= 0) { g.val[v] = EGS_TOP; g.ring[15] = 1; local++; }
+        if (h[v] >= 0) { g.val[v] = EGS_TOP; local++; }
     }
+    // one store of the TOP flag per workgroup (one per vertex queued on a single address)
+    if (__syncthreads_or(local > 0) && threadIdx.x == 0) g.ring[15] = 1;
     block_add(local, count);
 }
 
@@ -891,6 +918,16 @@ int gap_candidates(const int *win, const int *flo, int wmax, int mg, int64_t *ou
 // non-empty set is set to TOP (queued, not waited for).  has_big: some vertex is above
 // big_deg (its own kernels).  hist_ready: certify_prefetch was queued and has completed.  Returns 1 when vertices were
 // set to TOP (count in res.marked), 0 when no candidate gave a proof, < 0 on a HIP error.
+// A candidate's starting state in one launch: its flags, the pass counters and the upper
+// level of the first pass bitmap (the lower level is stored whole by the closure check).
+__global__ __launch_bounds__(256) void k_cert_clear(int *__restrict__ flags, int *__restrict__ cnt,
+                                                    unsigned *__restrict__ words, int64_t nwords) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4) flags[i] = 0;
+    if (i < kCntSlots) cnt[i] = 0;
+    for (int64_t k = i; k < nwords; k += (int64_t)gridDim.x * blockDim.x) words[k] = 0u;
+}
+
 hipError_t certify_prefetch(const DevGame &g, const CertBuffers &cb, int wmax, hipStream_t st) {
     hipError_t err = hipMemsetAsync(cb.hist, 0, sizeof(int) * (2 * kBins + 8), st);
     if (err != hipSuccess) return err;
@@ -938,9 +975,8 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
         const int B = (int)cands[c];
         res.threshold = B;
         res.tried++;
-        if (hipMemsetAsync(flags, 0, sizeof(int) * 4, st) != hipSuccess) return -1;
-        if (hipMemsetAsync(cb.bfcnt, 0, sizeof(int) * kCntSlots, st) != hipSuccess) return -1;
-        if (hipMemsetAsync(bm[0] + words, 0, sizeof(unsigned) * (bm_len - words), st) != hipSuccess) return -1;
+        hipLaunchKernelGGL(k_cert_clear, dim3(grid_for(std::max<int64_t>(bm_len - words, kCntSlots), 256, 64)),
+                           dim3(256), 0, st, flags, cb.bfcnt, bm[0] + words, bm_len - words);
         if (g.phases > 1) {
             const size_t lds = (size_t)g.phase_tiles * kSliceTile * (sizeof(long long) + sizeof(int));
             if (g.sshift)
