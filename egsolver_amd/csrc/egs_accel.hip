@@ -68,9 +68,16 @@ __global__ __launch_bounds__(256) void k_gap_hist(DevGame g, int wmax, int *__re
         for (int j = 0; j < kHistU; j++) {
             int bw = -1, bf = -1;
             if (x[j] != EGS_TOP && x[j] > 0) {
-                const unsigned q = ((unsigned)x[j] + w - 1u) / w;      // ceil(x / W) >= 1 (x, W < 2^31)
-                if ((q & (q - 1)) == 0) bw = 31 - __clz(q);
-                if ((unsigned)x[j] > w) bf = 31 - __clz(((unsigned)x[j] - 1u) / w);   // largest k with W 2^k < x
+                // k = the smallest k >= 0 with W 2^k >= x (no division: a leading-zero estimate
+                // corrected by comparisons).  ceil(x / W) = 2^k exactly when x > (2^k - 1) W, and
+                // for x > W the largest k' with W 2^k' < x is k - 1.
+                const int64_t xx = x[j], ww = w;
+                int k = (32 - __clz((unsigned)x[j] - 1u)) - (32 - __clz(w));
+                k = max(k, 0);
+                while (k > 0 && (ww << (k - 1)) >= xx) k--;
+                while ((ww << k) < xx) k++;
+                if (xx > ((1ll << k) - 1) * ww) bw = k;
+                if (xx > ww) bf = k - 1;
             }
             hist_add_wave(win, bw);
             hist_add_wave(flo, bf);
@@ -129,8 +136,14 @@ __device__ __forceinline__ bool bm_test(const unsigned *bm, int n, int u) {
     return (bm[u >> 5] >> (u & 31)) & 1u;
 }
 
-// Pass counters live in a ring of kCntSlots ints: pass p adds its raises to slot p % kCntSlots
-// and zeroes slot (p + 1) % kCntSlots for the pass after it.
+// Pass flags live in a ring of kCntSlots ints: pass p sets slot p % kCntSlots when it changes
+// a potential and zeroes slot (p + 1) % kCntSlots for the pass after it.  Only zero or not
+// is ever read, so a workgroup stores 1 instead of adding its count: thousands of
+// workgroups adding to one address as they finish together queue on it (a 2,048-workgroup
+// launch spent about 20 us on such adds).
+__device__ __forceinline__ void block_flag(bool any, int *dst) {
+    if (__syncthreads_or(any) && threadIdx.x == 0) *dst = 1;
+}
 constexpr int kCntSlots = 64;
 
 constexpr int kBigU = 4;   // edges in flight per lane in the workgroup-per-vertex kernels
@@ -147,7 +160,7 @@ constexpr int kBfU = 2;        // edges in flight per lane in k_bf_pass
 // (the first pass's result on D, else a sentinel), the changed vertices' bits into `bits`
 // (every word of the vertex range is stored; the summary must be zero on entry), zeroes
 // `clear` (the bitmap of the next pass),
-// counts changes into cnt[0], members into flags[3] and removals into flags[0].
+// sets cnt[0] when anything changed, counts members into flags[3] and removals into flags[0].
 __global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, long long bound,
                                                  long long *__restrict__ h, unsigned *__restrict__ bits,
                                                  unsigned *__restrict__ clear, int *__restrict__ cnt,
@@ -224,7 +237,7 @@ __global__ __launch_bounds__(256) void k_closure(DevGame g, int B, int last, lon
         clear[i] = 0u;
     block_add(members, &flags[3]);
     block_add(removed, &flags[0]);
-    block_add(changed, &cnt[0]);
+    block_flag(changed > 0, &cnt[0]);
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[1] = 0;
 }
 
@@ -365,7 +378,7 @@ __global__ __launch_bounds__(256) void k_closure_fused(DevGame g, int B, int las
         clear[i] = 0u;
     block_add(members, &flags[3]);
     block_add(removed, &flags[0]);
-    block_add(changed, &cnt[0]);
+    block_flag(changed > 0, &cnt[0]);
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[1] = 0;
 }
 
@@ -470,12 +483,12 @@ __global__ __launch_bounds__(256) void k_closure_big(DevGame g, int B, int last,
             if (owner1 ? !(s_flag & 1) : (s_flag & 2)) {
                 h[v] = kHOut;
                 bm_set(bits, g.n, v);
-                atomicAdd(&cnt[0], 1);
+                cnt[0] = 1;
                 atomicAdd(&flags[0], 1);
             } else if (b2 > h0) {
                 h[v] = b2;
                 bm_set(bits, g.n, v);
-                atomicAdd(&cnt[0], 1);
+                cnt[0] = 1;
                 if (b2 > bound) atomicOr(&flags[1], 1);
             }
         }
@@ -610,7 +623,7 @@ __device__ __forceinline__ void bf_pass_part(const DevGame &g, long long *__rest
             }
         }
     }
-    block_add(local, &cnt[p % kCntSlots]);
+    block_flag(local > 0, &cnt[p % kCntSlots]);
     block_add(removed, &flags[0]);
 }
 
@@ -684,7 +697,7 @@ __device__ __forceinline__ void bf_big_part(const DevGame &g, long long *__restr
             }
         }
     }
-    if (threadIdx.x == 0 && local) atomicAdd(&cnt[p % kCntSlots], local);
+    if (threadIdx.x == 0 && local) cnt[p % kCntSlots] = 1;
 }
 
 __global__ __launch_bounds__(256) void k_bf_big(DevGame g, long long *__restrict__ h, long long bound,
@@ -786,7 +799,7 @@ __device__ __forceinline__ void bf_huge_part(const DevGame &g, long long *__rest
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0 && local) atomicAdd(&cnt[p % kCntSlots], local);
+    if (threadIdx.x == 0 && local) cnt[p % kCntSlots] = 1;
     if (threadIdx.x == 0 && removed) atomicAdd(&flags[0], removed);
 }
 
@@ -830,14 +843,21 @@ hipError_t launch_hbest_reset(const DevGame &g, hipStream_t st) {
 }
 
 // D is where h >= 0 (set by k_closure for this candidate).
-__global__ __launch_bounds__(256) void k_mark_top(DevGame g, const long long *__restrict__ h, int *__restrict__ count) {
+__global__ __launch_bounds__(256) void k_mark_top(DevGame g, const long long *__restrict__ h) {
     int local = 0;
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
-        if (h[v] >= 0) { g.val[v] = EGS_TOP; local++; }
+    const int64_t threads = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v0 < g.n; v0 += 4 * threads) {
+        long long hv[4];                         // four potentials in flight per thread
+#pragma unroll
+        for (int k = 0; k < 4; k++) hv[k] = v0 + k * threads < g.n ? h[v0 + k * threads] : -1;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (hv[k] >= 0) { g.val[v0 + k * threads] = EGS_TOP; local++; }
     }
-    // one store of the TOP flag per workgroup (one per vertex queued on a single address)
+    // one store of the TOP flag per workgroup that marked a vertex (nothing counts the marks:
+    // the members left after the passes are the count, and a per-workgroup atomic on one
+    // counter serialises when the workgroups finish together)
     if (__syncthreads_or(local > 0) && threadIdx.x == 0) g.ring[15] = 1;
-    block_add(local, count);
 }
 
 static void launch_bf_all(int G, int grid, int big_grid, int huge_grid, hipStream_t st, const DevGame &g,
@@ -931,9 +951,10 @@ __global__ __launch_bounds__(256) void k_cert_clear(int *__restrict__ flags, int
 hipError_t certify_prefetch(const DevGame &g, const CertBuffers &cb, int wmax, hipStream_t st) {
     hipError_t err = hipMemsetAsync(cb.hist, 0, sizeof(int) * (2 * kBins + 8), st);
     if (err != hipSuccess) return err;
-    // few workgroups: each adds its non-empty bins to the global histogram with atomics
-    hipLaunchKernelGGL(k_gap_hist, dim3(std::min(grid_for(g.n, 256, 2048), 256)), dim3(256), 0, st, g, std::max(wmax, 1),
-                       cb.hist);
+    // each workgroup adds its non-empty bins to the global histogram with atomics (about 60
+    // addresses); 1024 workgroups keep four waves per SIMD for the per-value arithmetic
+    hipLaunchKernelGGL(k_gap_hist, dim3(std::min(grid_for(g.n, 256 * kHistU, 2048), 1024)), dim3(256), 0, st, g,
+                       std::max(wmax, 1), cb.hist);
     return hipMemcpyAsync(cb.pinned, cb.hist, sizeof(int) * 2 * kBins, hipMemcpyDeviceToHost, st);
 }
 
@@ -954,7 +975,7 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
     int64_t cands[41];
     int ncand = gap_candidates(cb.pinned, cb.pinned + kBins, wmax, g.mg, cands, std::min(max_cands, 40));
     cands[ncand++] = -1;                    // the vertices that rose in round `last`
-    int *flags = cb.hist + 2 * kBins;       // [0] removed, [1] bound exceeded, [2] marked, [3] members
+    int *flags = cb.hist + 2 * kBins;       // [0] removed, [1] bound exceeded, [2] unused, [3] members
     int *fl = cb.pinned + 2 * kBins;        // flags readback (4 ints), then the counter ring
     int *cnt = fl + 4;
     const int64_t words = ((int64_t)g.n + 31) >> 5, bm_len = words + (words + 31) / 32;
@@ -1028,7 +1049,7 @@ int certify_top(const DevGame &g, const CertBuffers &cb, int wmax, int last, int
         }
         res.marked = fl[3] - fl[0];         // members left after the removals
         if (res.marked <= 0) continue;      // pruning emptied the candidate: try the next
-        hipLaunchKernelGGL(k_mark_top, dim3(grid), dim3(256), 0, st, g, (const long long *)cb.h, flags + 2);
+        hipLaunchKernelGGL(k_mark_top, dim3(grid), dim3(256), 0, st, g, (const long long *)cb.h);
         return hipGetLastError() == hipSuccess ? 1 : -1;
     }
     return 0;
@@ -1067,7 +1088,7 @@ __global__ __launch_bounds__(256) void k_cert_all_seed(DevGame g, long long *__r
         cur[i] = 0u;
     }
     block_add(members, &flags[3]);
-    block_add(members, &cnt[0]);
+    block_flag(members > 0, &cnt[0]);
 }
 
 // Potentials for export: -1 outside the certified set.
