@@ -632,6 +632,13 @@ __global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int j) {
 // decides k_init's initial activity of the others into stamp.
 constexpr int kSliceQ = kSliceTile + 8;   // counter row per slice: rows 8 banks apart
 
+// the slice set-up's reads of the CSR edges: nontemporal, so they do not push the slice
+// edges just written out of the memory-side cache before the first round reads them
+__device__ __forceinline__ int2 edge_nt(const int2 *e, int64_t k) {
+    const long long x = __builtin_nontemporal_load((const long long *)e + k);
+    return make_int2((int)(x & 0xffffffffll), (int)(x >> 32));
+}
+
 template <int G>
 __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *__restrict__ slen,
                                                            int *__restrict__ sbase, void *__restrict__ sedges,
@@ -667,7 +674,7 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
         for (int i = 0; i < G; i++) {
             const int vl = i * kGroups + grp;
             const int k = s_beg[vl] + sub;
-            ew[i] = k < s_end[vl] ? g.edges[k] : make_int2(-1, 0);
+            ew[i] = k < s_end[vl] ? edge_nt(g.edges, k) : make_int2(-1, 0);
         }
 #pragma unroll
         for (int i = 0; i < G; i++) {
@@ -679,7 +686,7 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
             const int vl = i * kGroups + grp;
             const int e = s_end[vl];
             for (int k = s_beg[vl] + G + sub; k < e; k += G) {
-                const int2 x = g.edges[k];
+                const int2 x = edge_nt(g.edges, k);
                 atomicAdd(&s_cnt[(x.x >> g.slice_shift) * kSliceQ + vl], 1);
                 (x.y < 0 ? neg : nonneg) |= 1u << i;
             }
@@ -731,7 +738,7 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
         for (int i = 0; i < G; i++) {
             const int vl = i * kGroups + grp;
             const int e = s_end[vl];
-            for (int k = s_beg[vl] + G + sub; k < e; k += G) put(g.edges[k], vl);
+            for (int k = s_beg[vl] + G + sub; k < e; k += G) put(edge_nt(g.edges, k), vl);
         }
         __syncthreads();    // s_wt, s_beg, s_end and s_cnt are rewritten for the next tile
     }
