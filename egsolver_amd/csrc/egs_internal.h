@@ -31,7 +31,8 @@ struct DevGame {
     int *list[2];         // output lists, ping-pong: kSub*sub_cap + n slots each
     int sub_cap;
     int *lcnt;            // 3 slots x kListCounters
-    int *mode;            // 3 slots: mode of round r at r % 3
+    int *mode;            // [0..2] mode of round r at r % 3; [3] a round whose input list
+                          // holds its count but no items (a phased dense round before the transpose)
     int *dbig, *dhuge;    // per-round deferral lists (sparse modes)
     int *ring;            // [3..5] dbig counters, [6..8] dhuge counters, [9] nsbig, [10] nshuge,
                           // [11] a round that must sweep all vertices (after a TOP proof),

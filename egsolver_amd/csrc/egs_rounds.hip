@@ -142,9 +142,11 @@ __device__ __forceinline__ int decide_mode(const DevGame &g, int r, int total) {
 }
 
 // The sparse modes walk predecessor lists: until the transpose exists (ring[12], raised by
-// the host) such a round sweeps densely instead and counts itself in ring[13].
-__device__ __forceinline__ int gate_mode(const DevGame &g, int mode) {
-    return (mode == MODE_QUEUE || mode == MODE_LIFTED) && g.ring[12] == 0 ? MODE_DENSE : mode;
+// the host) such a round sweeps densely instead and counts itself in ring[13].  They also
+// read the input list's items: a round whose input holds only the count (mode[3] == r) sweeps
+// densely too.
+__device__ __forceinline__ int gate_mode(const DevGame &g, int mode, int r) {
+    return (mode == MODE_QUEUE || mode == MODE_LIFTED) && (g.ring[12] == 0 || g.mode[3] == r) ? MODE_DENSE : mode;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -479,8 +481,8 @@ __global__ __launch_bounds__(256) void k_round(DevGame g, int j) {
     __syncthreads();
     if (threadIdx.x == 0) {
         const int want = decide_mode(g, r, pre[kSub + 1]);
-        smode = gate_mode(g, want);
-        if (blockIdx.x == 0 && smode != want) g.ring[13]++;     // rounds that wanted the transpose
+        smode = gate_mode(g, want, r);
+        if (blockIdx.x == 0 && smode != want && g.ring[12] == 0) g.ring[13]++;   // rounds that wanted the transpose
     }
     __syncthreads();
     const int mode = smode;
@@ -541,6 +543,9 @@ __global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int j) {
     __syncthreads();
     const Stage st{sbuf, &scnt, g.list[(r + 1) & 1], g.lcnt + ((r + 1) % 3) * kListCounters,
                    (int)(blockIdx.x % kSub)};
+    // Until the transpose exists the next round can only sweep densely, which reads the
+    // output list's count and not its items: then the risers are counted, not written.
+    const bool items = g.ring[12] != 0;
     LiftAcc acc;
     const int P = g.phases;
     const int64_t tiles = ((int64_t)g.n + kSliceTile - 1) / kSliceTile;
@@ -612,10 +617,24 @@ __global__ __launch_bounds__(256) void k_dense_fused(DevGame g, int j) {
                 lifted = true;
             }
         }
-        stage_push(g, st, lifted, v);
-        if ((k & 7) == 7) stage_flush(g, st, kLdsCap / 2);
+        if (items) {
+            stage_push(g, st, lifted, v);
+            if ((k & 7) == 7) stage_flush(g, st, kLdsCap / 2);
+        } else {
+            const uint64_t mask = __ballot(lifted);
+            if (egs_lane() == 0 && mask) atomicAdd(&scnt, __popcll(mask));
+        }
     }
-    stage_flush(g, st, 1);
+    if (items) {
+        stage_flush(g, st, 1);
+    } else {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int bs, ns, bo;
+            if (scnt > 0) reserve_split(g, st.cnts, st.sub, scnt, bs, ns, bo);
+            if (blockIdx.x == 0) g.mode[3] = r + 1;      // round r+1's input: the count only
+        }
+    }
     acc.edges = edges;
     flush_stats(g, acc);
 }

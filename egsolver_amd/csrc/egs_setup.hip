@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void k_prepare_reset(DevGame g) {
         g.stats[i] = 0;
     if (blockIdx.x == 0) {
         for (int i = threadIdx.x; i < 3 * kListCounters; i += blockDim.x) g.lcnt[i] = 0;
-        if (threadIdx.x < 3) g.mode[threadIdx.x] = MODE_QUEUE;
+        if (threadIdx.x < 4) g.mode[threadIdx.x] = threadIdx.x < 3 ? MODE_QUEUE : 0;
         if (threadIdx.x < 16) g.ring[threadIdx.x] = threadIdx.x == 14 ? 1 : 0;
         if (threadIdx.x == 0) *g.rounds_done = 0;
     }
