@@ -55,6 +55,14 @@ __device__ __forceinline__ void hist_add_wave(int *bins, int bin) {
 constexpr int kHistU = 8;   // values in flight per thread
 
 __global__ __launch_bounds__(256) void k_gap_hist(DevGame g, int wmax, int *__restrict__ out) {
+    {   // the next round's input counters (round ring[14]): an empty frontier ends the run at
+        // this pause before any candidate is read, so no histogram is needed.  Every wave
+        // reads the same counters and leaves together.
+        const int *c = g.lcnt + (g.ring[14] % 3) * kListCounters;
+        const int lane = threadIdx.x & 63;
+        const bool any = c[lane] != 0 || (lane == 0 && c[kSub] != 0);
+        if (__ballot(any) == 0) return;
+    }
     __shared__ int win[kBins], flo[kBins];
     for (int i = threadIdx.x; i < kBins; i += blockDim.x) { win[i] = 0; flo[i] = 0; }
     __syncthreads();
@@ -842,8 +850,10 @@ hipError_t launch_hbest_reset(const DevGame &g, hipStream_t st) {
     return hipGetLastError();
 }
 
-// D is where h >= 0 (set by k_closure for this candidate).
+// D is where h >= 0 (set by k_closure for this candidate).  The values were raised outside
+// the rounds, so the next round (ring[14]) sweeps every vertex: ring[11] names it.
 __global__ __launch_bounds__(256) void k_mark_top(DevGame g, const long long *__restrict__ h) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) g.ring[11] = g.ring[14];
     int local = 0;
     const int64_t threads = (int64_t)gridDim.x * blockDim.x;
     for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v0 < g.n; v0 += 4 * threads) {

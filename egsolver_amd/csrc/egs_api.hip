@@ -55,7 +55,8 @@ int64_t egs_max_loop_c(int64_t m, int64_t n, int64_t mg, int64_t maxloop) {
 }
 
 // Offsets (in ints) inside the solver's pinned readback block.
-constexpr int kPinProbe = 0;      // 2 x 128: per-batch counter probes
+constexpr int kPinProbe = 2048;   // 2 x kProbeStride: per-batch counter probes
+constexpr int kProbeStride = 512;
 constexpr int kPinDeg = 256;
 constexpr int kPinInit = 320;     // kListCounters ints
 constexpr int kPinConv = 448;     // kListCounters ints
@@ -63,9 +64,8 @@ constexpr int kPinRounds = 576;
 constexpr int kPinStats = 640;    // 4 x u64
 constexpr int kPinViol = 704;     // 1 x u64
 constexpr int kPinCert = 1024;    // certification readbacks (2 x 64 + 4 + 64 ints)
-constexpr int kPinForce = 1280;   // round number forced to a full sweep
 constexpr int kPinOne = 1281;     // the constant 1 (raises ring[12])
-constexpr int kProbeWant = 100;   // offset in a probe slot: ring[13] (rounds that wanted to go sparse)
+constexpr int kProbeWant = 100;   // offset in a probe slot of ring[13] when it needs a copy of its own
 // The transpose costs about ten dense rounds to build, and a sparse round saves at most one
 // dense round: build it once this many rounds have wanted it (a proof usually leaves only a
 // round or two, which do not pay for it).
@@ -201,6 +201,7 @@ struct egs_solver {
     int2 *d_edges = nullptr;
     unsigned long long *d_stats = nullptr, *d_statsum = nullptr;
     int *h_probe = nullptr;   // pinned: per-batch counter probes + misc readbacks
+    int probe_want[2] = {kProbeWant, kProbeWant};   // where each probe slot holds ring[13]
     hipEvent_t ev_probe[2] = {nullptr, nullptr};
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
 
@@ -429,7 +430,7 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
         return fail(EGS_E_NOMEM, "hipMalloc of %zu bytes failed", L.total);
     }
     bind_layout(s, (char *)s->block, L);
-    if (hipHostMalloc((void **)&s->h_probe, 8192, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&s->h_probe, 4 * (kPinProbe + 2 * kProbeStride), hipHostMallocDefault) != hipSuccess) {
         egs_solver_destroy(s);
         return fail(EGS_E_NOMEM, "hipHostMalloc failed");
     }
@@ -853,10 +854,7 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
             if (got > 0) {
                 s->st.certificates++;
                 s->st.top_marked += cr.marked;
-                // next round sweeps everything (the slot is rewritten only after later waits)
-                s->h_probe[kPinForce] = (int)s->next_round;
-                HIP_TRY(hipMemcpyAsync(s->d_ring + 11, s->h_probe + kPinForce, sizeof(int), hipMemcpyHostToDevice,
-                                       s->stream));
+                // the marking kernel has named the next round (ring[11]) a full sweep
                 // the spacing only grows (doubled by failures): a long run that keeps
                 // proving small sets does not pay an attempt every few rounds
                 batch = 1;        // the fixpoint often follows at once: probe again early
@@ -889,18 +887,28 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
         s->next_round += k;
         launched += k;
         const int slot = (pending + 1) & 1;
-        HIP_TRY(hipMemcpyAsync(s->h_probe + kPinProbe + slot * 128,
-                               s->d_lcnt + (s->next_round % 3) * egs::kListCounters,
-                               sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
-        if (!s->csc_built)
-            HIP_TRY(hipMemcpyAsync(s->h_probe + kPinProbe + slot * 128 + kProbeWant, s->d_ring + 13, sizeof(int),
-                                   hipMemcpyDeviceToHost, s->stream));
+        int *hp = s->h_probe + kPinProbe + slot * kProbeStride;
+        const int *src = s->d_lcnt + (s->next_round % 3) * egs::kListCounters;
+        // Until the transpose exists the probe also reads ring[13].  The list counters, the
+        // round modes and the ring lie in that order in the device arena, so one copy of the
+        // span from this slot's counters to ring[13] reads both.
+        const ptrdiff_t span = s->d_ring + 14 - src;
+        if (s->csc_built) {
+            HIP_TRY(hipMemcpyAsync(hp, src, sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
+        } else if (span > egs::kListCounters && span <= kProbeStride) {
+            HIP_TRY(hipMemcpyAsync(hp, src, sizeof(int) * (size_t)span, hipMemcpyDeviceToHost, s->stream));
+            s->probe_want[slot] = (int)span - 1;
+        } else {
+            HIP_TRY(hipMemcpyAsync(hp, src, sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
+            HIP_TRY(hipMemcpyAsync(hp + kProbeWant, s->d_ring + 13, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+            s->probe_want[slot] = kProbeWant;
+        }
         HIP_TRY(probe_mark(s, slot));
         if (pending >= 0) {
             HIP_TRY(probe_wait(s, pending));
-            if (list_total(s, s->h_probe + kPinProbe + pending * 128) == 0) done = true;
+            if (list_total(s, s->h_probe + kPinProbe + pending * kProbeStride) == 0) done = true;
             // enough rounds wanted to go sparse: build the transpose now
-            if (!done && !s->csc_built && s->h_probe[kPinProbe + pending * 128 + kProbeWant] >= kSparseWantRounds)
+            if (!done && !s->csc_built && s->h_probe[kPinProbe + pending * kProbeStride + s->probe_want[pending]] >= kSparseWantRounds)
                 HIP_TRY(build_csc(s));
         }
         pending = slot;
