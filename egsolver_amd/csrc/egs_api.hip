@@ -756,8 +756,10 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     if (s->sh.use_block)             // shared lifts of the vertices above huge_deg start empty
         HIP_TRY(egs::launch_hbest_reset(s->g, s->stream));
     HIP_TRY(egs::launch_classify(s->g, s->d_ring + 9, s->stream));
-    HIP_TRY(egs::launch_slices(s->g, s->stream));
-    HIP_TRY(egs::launch_init(s->g, s->stream));
+    // with slices and no transpose the slice set-up also stores the initial state
+    const bool init_in_slices = s->g.phases > 1 && !s->csc_built;
+    HIP_TRY(egs::launch_slices(s->g, s->stream, init_in_slices));
+    HIP_TRY(egs::launch_init(s->g, s->stream, init_in_slices));
     // the initial frontier's counters ride on the same wait as the timing
     HIP_TRY(hipMemcpyAsync(s->h_probe + kPinInit, s->d_lcnt + 1 * egs::kListCounters,
                            sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));

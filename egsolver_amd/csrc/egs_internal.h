@@ -131,8 +131,10 @@ hipError_t launch_read_order_check(const int *rptr, const int *rsucc, int n, int
 hipError_t launch_internalise(const ReadOrderBuffers &b, int n, int *ptr, int2 *edges, unsigned long long *mg,
                               hipStream_t st);
 hipError_t launch_classify(const DevGame &g, int *counts, hipStream_t st);
-hipError_t launch_init(const DevGame &g, hipStream_t st);
-hipError_t launch_slices(const DevGame &g, hipStream_t st);
+// in_slices: the slice set-up (init) already stored the initial state of the vertices up to
+// big_deg and counted, without listing, the active ones (no transpose: round 1 sweeps densely)
+hipError_t launch_init(const DevGame &g, hipStream_t st, bool in_slices);
+hipError_t launch_slices(const DevGame &g, hipStream_t st, bool init);
 int pack_shift(int n, int wmax);
 // Round j of a batch (absolute round ring[14] + j); launch_advance ends the batch.
 hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int j, hipStream_t st);

@@ -661,7 +661,7 @@ __device__ __forceinline__ int2 edge_nt(const int2 *e, int64_t k) {
 template <int G>
 __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *__restrict__ slen,
                                                            int *__restrict__ sbase, void *__restrict__ sedges,
-                                                           unsigned long long *__restrict__ bigmask) {
+                                                           unsigned long long *__restrict__ bigmask, int init) {
     constexpr int kGroups = kSliceTile / G;     // vertices one step of the workgroup covers
     __shared__ int s_cnt[kMaxPhases * kSliceQ];  // [slice][vertex of the tile]
     __shared__ int s_beg[kSliceTile], s_end[kSliceTile];
@@ -670,6 +670,7 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int sub = threadIdx.x & (G - 1), grp = threadIdx.x / G;
     const int64_t tiles = ((int64_t)g.n + kSliceTile - 1) / kSliceTile;
+    int active = 0;                               // init: this workgroup's initially active vertices
     for (int64_t T = blockIdx.x; T < tiles; T += gridDim.x) {
         const int64_t first = T * kSliceTile;
         const int64_t v = first + threadIdx.x;
@@ -680,6 +681,11 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
             ve = g.ptr[v + 1];
         }
         const bool big = valid && ve - vs > g.big_deg;
+        if (init && valid) {                      // k_init's stores (k_init_big decides the big ones)
+            g.val[v] = 0;
+            if (g.rise) g.rise[v] = 0;
+            if (big) g.stamp[v] = 0;
+        }
         const unsigned long long word = __ballot(big);
         if (lane == 0 && valid) bigmask[v >> 6] = word;
         s_beg[threadIdx.x] = vs;
@@ -719,8 +725,12 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
             const int vl = sub * kGroups + grp;
             const int64_t u = first + vl;
             // lift(0)(u) > 0: owner 0 with every edge negative, owner 1 with a negative edge
-            if (u < g.n && s_end[vl] >= s_beg[vl])
-                g.stamp[u] = ((u < g.n0) ? !((nonneg >> sub) & 1) : ((neg >> sub) & 1)) ? 1 : 0;
+            bool act = false;
+            if (u < g.n && s_end[vl] >= s_beg[vl]) {
+                act = (u < g.n0) ? !((nonneg >> sub) & 1) : ((neg >> sub) & 1);
+                g.stamp[u] = act ? 1 : 0;
+            }
+            if (init) active += __syncthreads_count(act);
         }
         __syncthreads();
         uint8_t *len_out = slen + (size_t)T * P * kSliceTile;
@@ -760,6 +770,12 @@ __global__ __launch_bounds__(kSliceTile) void k_slice_build(DevGame g, uint8_t *
             for (int k = s_beg[vl] + G + sub; k < e; k += G) put(edge_nt(g.edges, k), vl);
         }
         __syncthreads();    // s_wt, s_beg, s_end and s_cnt are rewritten for the next tile
+    }
+    // init: round 1's input is only counted (without the transpose round 1 sweeps densely)
+    if (init && threadIdx.x == 0) {
+        int bs, ns, bo;
+        if (active > 0) reserve_split(g, g.lcnt + 1 * kListCounters, (int)(blockIdx.x % kSub), active, bs, ns, bo);
+        if (blockIdx.x == 0) g.mode[3] = 1;
     }
 }
 
@@ -918,8 +934,8 @@ __global__ __launch_bounds__(256) void k_lift_hubs(DevGame g, int j, int wave_gr
 // ---------------------------------------------------------------------------------------
 
 // The counters, modes and ring slots start cleared (k_prepare_reset).
-hipError_t launch_init(const DevGame &g, hipStream_t st) {
-    hipLaunchKernelGGL(k_init, dim3(grid_for(g.n, 256, 2048)), dim3(256), 0, st, g);
+hipError_t launch_init(const DevGame &g, hipStream_t st, bool in_slices) {
+    if (!in_slices) hipLaunchKernelGGL(k_init, dim3(grid_for(g.n, 256, 2048)), dim3(256), 0, st, g);
     hipLaunchKernelGGL(k_init_big, dim3(1024), dim3(256), 0, st, g);
     return hipGetLastError();
 }
@@ -969,8 +985,9 @@ hipError_t launch_round(const DevGame &g, const LaunchShape &sh, int j, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_slices(const DevGame &g, hipStream_t st) {
+hipError_t launch_slices(const DevGame &g, hipStream_t st, bool init) {
     if (g.phases <= 1) return hipSuccess;
+    const int in = init ? 1 : 0;
     // lanes per vertex: the average degree rounded up to 4, 8 or 16
     const int64_t avg = (g.m + g.n - 1) / g.n;
     const dim3 grid(grid_for(g.n, kSliceTile, 4096));
@@ -979,11 +996,11 @@ hipError_t launch_slices(const DevGame &g, hipStream_t st) {
     void *sedges = const_cast<void *>(g.sedges);
     unsigned long long *bigmask = const_cast<unsigned long long *>(g.bigmask);
     if (avg <= 4)
-        hipLaunchKernelGGL(k_slice_build<4>, grid, dim3(kSliceTile), 0, st, g, slen, sbase, sedges, bigmask);
+        hipLaunchKernelGGL(k_slice_build<4>, grid, dim3(kSliceTile), 0, st, g, slen, sbase, sedges, bigmask, in);
     else if (avg <= 8)
-        hipLaunchKernelGGL(k_slice_build<8>, grid, dim3(kSliceTile), 0, st, g, slen, sbase, sedges, bigmask);
+        hipLaunchKernelGGL(k_slice_build<8>, grid, dim3(kSliceTile), 0, st, g, slen, sbase, sedges, bigmask, in);
     else
-        hipLaunchKernelGGL(k_slice_build<16>, grid, dim3(kSliceTile), 0, st, g, slen, sbase, sedges, bigmask);
+        hipLaunchKernelGGL(k_slice_build<16>, grid, dim3(kSliceTile), 0, st, g, slen, sbase, sedges, bigmask, in);
     return hipGetLastError();
 }
 
