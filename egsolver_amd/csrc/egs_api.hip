@@ -60,8 +60,7 @@ constexpr int kProbeStride = 512;
 constexpr int kPinDeg = 256;
 constexpr int kPinInit = 320;     // kListCounters ints
 constexpr int kPinConv = 448;     // kListCounters ints
-constexpr int kPinRounds = 576;
-constexpr int kPinStats = 640;    // 4 x u64
+constexpr int kPinStats = 640;    // 6 x u64
 constexpr int kPinViol = 704;     // 1 x u64
 constexpr int kPinCert = 1024;    // certification readbacks (2 x 64 + 4 + 64 ints)
 constexpr int kPinOne = 1281;     // the constant 1 (raises ring[12])
@@ -781,10 +780,8 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
 static int queue_stats(egs_solver *s) {
     HIP_TRY(egs::launch_sum_stats(s->g, egs::kStatSlots, s->d_statsum, s->stream));
     unsigned long long *h = (unsigned long long *)(s->h_probe + kPinStats);
-    HIP_TRY(hipMemcpyAsync(h, s->d_statsum, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->h_probe + kPinRounds, s->d_rounds, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->h_probe + kPinConv, s->d_lcnt + (s->next_round % 3) * egs::kListCounters,
-                           sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
+    // statistics, last round and the next round's input size in one read-back
+    HIP_TRY(hipMemcpyAsync(h, s->d_statsum, sizeof(unsigned long long) * 6, hipMemcpyDeviceToHost, s->stream));
     return EGS_OK;
 }
 
@@ -794,9 +791,9 @@ static int collect_stats(egs_solver *s) {
     s->st.edges_relaxed = (int64_t)h[1];
     s->st.lifted = (int64_t)h[2];
     s->st.pred_visits = (int64_t)h[3];
-    s->st.phases = s->h_probe[kPinRounds];
+    s->st.phases = (int32_t)h[4];
     s->st.csc_built = s->csc_built ? 1 : 0;
-    s->st.converged = list_total(s, s->h_probe + kPinConv) == 0;
+    s->st.converged = h[5] == 0;
     return EGS_OK;
 }
 

@@ -252,9 +252,19 @@ __global__ void k_degree_max(const int *__restrict__ ptr, int n, int *__restrict
     if ((threadIdx.x & 63) == 0) atomicMax(out, mx);
 }
 
-__global__ __launch_bounds__(1024) void k_sum_stats(const unsigned long long *__restrict__ slots, int nslots,
-                                                    unsigned long long *__restrict__ out) {
+// out[0..3]: the summed statistics; out[4]: the last round that ran; out[5]: the size of the
+// next round's input list (round ring[14]; 0 = the least fixpoint), so that one read-back
+// closes a run.
+__global__ __launch_bounds__(1024) void k_sum_stats(DevGame g, const unsigned long long *__restrict__ slots,
+                                                    int nslots, unsigned long long *__restrict__ out) {
     __shared__ unsigned long long part[4][16];
+    if (threadIdx.x == 0) {
+        const int *c = g.lcnt + (g.ring[14] % 3) * kListCounters;
+        long long t = c[kSub];
+        for (int k = 0; k < kSub; k++) t += min(c[k], g.sub_cap);
+        out[4] = (unsigned long long)*g.rounds_done;
+        out[5] = (unsigned long long)t;
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     unsigned long long s[4] = {0, 0, 0, 0};
     for (int b = threadIdx.x; b < nslots; b += blockDim.x) {
@@ -377,7 +387,7 @@ hipError_t launch_degree_stats(const int *ptr, int n, int *out, hipStream_t st) 
 }
 
 hipError_t launch_sum_stats(const DevGame &g, int nslots, unsigned long long *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_sum_stats, dim3(1), dim3(1024), 0, st, g.stats, nslots, out);
+    hipLaunchKernelGGL(k_sum_stats, dim3(1), dim3(1024), 0, st, g, g.stats, nslots, out);
     return hipGetLastError();
 }
 
