@@ -203,6 +203,8 @@ struct egs_solver {
     int probe_want[2] = {kProbeWant, kProbeWant};   // where each probe slot holds ring[13]
     hipEvent_t ev_probe[2] = {nullptr, nullptr};
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    hipEvent_t ev_pa = nullptr, ev_pb = nullptr;   // prepare's span, read after a later wait
+    bool prep_open = false;   // prepare's time and initial frontier not read yet
 
     egs::DevGame g{};
     egs::LaunchShape sh{};
@@ -297,6 +299,19 @@ static int64_t list_total(const egs_solver *s, const int *c) {
     int64_t t = 0;
     for (int k = 0; k < egs::kSub; k++) t += std::min(c[k], s->sub_cap);
     return t + c[egs::kSub];
+}
+
+// prepare's time and initial frontier size, once its closing event has passed
+static hipError_t close_prepare(egs_solver *s) {
+    if (!s->prep_open) return hipSuccess;
+    float ms = 0.f;
+    hipError_t e = hipEventSynchronize(s->ev_pb);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, s->ev_pa, s->ev_pb);
+    if (e != hipSuccess) return e;
+    s->st.ms_prepare = ms;
+    s->st.initial_active = (int32_t)list_total(s, s->h_probe + kPinInit);
+    s->prep_open = false;
+    return hipSuccess;
 }
 
 static void drop_graphs(egs_solver *s) {
@@ -403,6 +418,8 @@ int egs_solver_destroy(egs_solver *s) {
     for (auto &e : s->ev_probe) if (e) (void)hipEventDestroy(e);
     if (s->ev_a) (void)hipEventDestroy(s->ev_a);
     if (s->ev_b) (void)hipEventDestroy(s->ev_b);
+    if (s->ev_pa) (void)hipEventDestroy(s->ev_pa);
+    if (s->ev_pb) (void)hipEventDestroy(s->ev_pb);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return EGS_OK;
@@ -436,7 +453,8 @@ int egs_solver_create(int device, int32_t n_cap, int64_t m_cap, egs_solver **out
     s->h_probe[kPinOne] = 1;
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&s->ev_probe[0]) != hipSuccess || hipEventCreate(&s->ev_probe[1]) != hipSuccess ||
-        hipEventCreate(&s->ev_a) != hipSuccess || hipEventCreate(&s->ev_b) != hipSuccess) {
+        hipEventCreate(&s->ev_a) != hipSuccess || hipEventCreate(&s->ev_b) != hipSuccess ||
+        hipEventCreate(&s->ev_pa) != hipSuccess || hipEventCreate(&s->ev_pb) != hipSuccess) {
         egs_solver_destroy(s);
         return fail(EGS_E_HIP, "stream/event creation failed");
     }
@@ -743,8 +761,14 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
         memcpy(&s->graph_g, &s->g, sizeof(s->g));
         memcpy(&s->graph_sh, &s->sh, sizeof(s->sh));
     }
+    // The set-up is timed by its own events and not waited for: the first round queues
+    // behind it at once, and its time and the initial frontier are read after a later wait.
+    // (The no-allocation solver has no events: it times on the host and waits here.)
+    HIP_TRY(close_prepare(s));
+    const bool lazy = s->ev_pa && s->ev_pb;
     double t0 = 0;
-    HIP_TRY(mark_start(s, &t0));
+    if (lazy) HIP_TRY(hipEventRecord(s->ev_pa, s->stream));
+    else HIP_TRY(mark_start(s, &t0));
     HIP_TRY(egs::launch_prepare_reset(s->g, s->stream));   // statistics, counters, modes, ring slots
     // The transpose only feeds the sparse round modes, and many solves never need them
     // (config-2-like games sweep densely until a proof ends them).  Rounds stay dense while
@@ -762,11 +786,17 @@ int egs_solver_prepare(egs_solver *s, const egs_options *opt) {
     // the initial frontier's counters ride on the same wait as the timing
     HIP_TRY(hipMemcpyAsync(s->h_probe + kPinInit, s->d_lcnt + 1 * egs::kListCounters,
                            sizeof(int) * egs::kListCounters, hipMemcpyDeviceToHost, s->stream));
-    float ms = 0.f;
-    HIP_TRY(mark_stop(s, t0, &ms));
-    memset(&s->st, 0, sizeof(s->st));
-    s->st.ms_prepare = ms;
-    s->st.initial_active = (int32_t)list_total(s, s->h_probe + kPinInit);
+    if (lazy) {
+        HIP_TRY(hipEventRecord(s->ev_pb, s->stream));
+        memset(&s->st, 0, sizeof(s->st));
+        s->prep_open = true;
+    } else {
+        float ms = 0.f;
+        HIP_TRY(mark_stop(s, t0, &ms));
+        memset(&s->st, 0, sizeof(s->st));
+        s->st.ms_prepare = ms;
+        s->st.initial_active = (int32_t)list_total(s, s->h_probe + kPinInit);
+    }
     s->next_round = 1;
     s->cert_gap = s->cert_every;
     s->next_cert = s->cert_every;
@@ -918,6 +948,7 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
     if (rc) return rc;
     float ms = 0.f;
     HIP_TRY(mark_stop(s, t0, &ms));                  // also the wait for the statistics
+    HIP_TRY(close_prepare(s));
     rc = collect_stats(s);
     if (rc) return rc;
     s->st.ms_iterate += ms;
@@ -927,6 +958,7 @@ int egs_solver_iterate(egs_solver *s, int64_t max_phases, egs_stats *st) {
 
 int egs_solver_stats(egs_solver *s, egs_stats *st) {
     if (!s || !st) return fail(EGS_E_ARG, "null argument");
+    HIP_TRY(close_prepare(s));
     *st = s->st;
     return EGS_OK;
 }
@@ -1035,7 +1067,7 @@ constexpr int kNoAllocDevices = 64;
 
 struct NoAllocSlot {
     egs_solver s;
-    int probe[2048];     // the solver's read-back area (pageable; every read waits on the stream)
+    int probe[kPinProbe + 2 * kProbeStride];   // the solver's read-back area (pageable; every read waits on the stream)
     bool ready = false;
 };
 NoAllocSlot g_noalloc[kNoAllocDevices];
